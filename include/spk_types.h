@@ -1,0 +1,102 @@
+/*
+ * spk_types.h -- plain-old-data descriptors shared by the MI355X path tracer's
+ * C ABI (spk.h) and its CPU checker.
+ *
+ * Every struct here is a flat description of what spica's scene description
+ * hands to PathIntegrator::render (reference: sources/spica/sceneparser.cc:190-335
+ * builds primitives, materials and lights; sources/core/integrator.cc:46-110 reads
+ * the camera, film and "sampleCount"; sources/integrators/path/path.cc:68 reads
+ * "maxDepth").  No C++ types, no torch types: pointers and sizes only.
+ */
+#ifndef SPK_TYPES_H_
+#define SPK_TYPES_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material kinds: the BSDF plugins on the path.
+ *   DIFFUSE        bsdfs/diffuse.cc:23-32        LambertianReflection(clamp(reflectance))
+ *   CONDUCTOR      bsdfs/conductor.cc:28-38      SpecularReflection + FresnelConductor(1, eta, k)
+ *   DIELECTRIC     bsdfs/dielectric.cc:30-42     FresnelSpecular(re, tr, 1, gray(intIOR))
+ *   ROUGHCONDUCTOR bsdfs/roughconductor.cc:38-75 MicrofacetReflection + GGX(alpha) + FresnelConductor
+ * Spica requires every shape to carry a BSDF (core/material.cc:21-26 dereferences it
+ * unconditionally), so there is no "no material" kind. */
+enum spk_material_type {
+    SPK_MAT_DIFFUSE = 1,
+    SPK_MAT_CONDUCTOR = 2,
+    SPK_MAT_DIELECTRIC = 3,
+    SPK_MAT_ROUGHCONDUCTOR = 4
+};
+
+typedef struct spk_material {
+    int32_t type;             /* enum spk_material_type */
+    int32_t reserved;
+    double reflectance[3];    /* diffuse "reflectance"; dielectric "specularReflectance" */
+    double transmittance[3];  /* dielectric "specularTransmittance" */
+    double eta[3];            /* (rough)conductor "eta" */
+    double k[3];              /* (rough)conductor "k" */
+    double int_ior;           /* dielectric "intIOR" (spica default 1.333) */
+    double alpha;             /* roughconductor "alpha" (spica default 0.1); 0 -> specular */
+} spk_material;
+
+/* Scene: a triangle soup in world space (spica turns every mesh into one Triangle
+ * primitive per face, sceneparser.cc:257-270), each triangle with a material and,
+ * when emissive, its own area light (sceneparser.cc:173-179).  Light order is the
+ * order of area_tri, followed by the environment map when present. */
+typedef struct spk_scene_desc {
+    int64_t num_tris;
+    const float* verts;          /* [num_tris][3][3] world-space vertex positions */
+    const float* normals;        /* [num_tris][3][3] per-vertex normals or NULL (face normals) */
+    const int32_t* mat_ids;      /* [num_tris] index into materials */
+    const int32_t* light_ids;    /* [num_tris] index into area lights or -1 */
+    int32_t num_materials;
+    int32_t num_area_lights;
+    const spk_material* materials;
+    const int32_t* area_tri;     /* [num_area_lights] the emitting triangle */
+    const double* area_radiance; /* [num_area_lights][3] "radiance" */
+    /* Environment map (lights/envmap.cc:17-55); env_width == 0 means none. */
+    int32_t env_width;
+    int32_t env_height;
+    const float* env_rgb;        /* [env_height][env_width][3], row y = theta bin */
+    double env_to_world[16];     /* row-major "toWorld"; spica stores its transpose */
+    double env_scale;            /* "scale" */
+    double env_center[3];        /* "worldCenter" */
+    double env_radius;           /* "worldRadius" (spica default 2.0) */
+} spk_scene_desc;
+
+/* Perspective camera (cameras/perspective.cc:23-74): "toWorld", "fov" (degrees,
+ * vertical), "apertureRadius", "focusDistance", film "width"/"height". */
+typedef struct spk_camera_desc {
+    int32_t width;
+    int32_t height;
+    double to_world[16];         /* row-major camera-to-world (usually a lookAt) */
+    double fov_deg;
+    double aperture_radius;
+    double focus_distance;
+} spk_camera_desc;
+
+enum spk_sampler_type {
+    SPK_SAMPLER_INDEPENDENT = 0, /* counter-based PCG4D uniform per (pixel, sample, dim) */
+    SPK_SAMPLER_SOBOL = 1,       /* shuffled, Owen-scrambled 2D Sobol per dimension pair */
+    SPK_SAMPLER_MT19937 = 2      /* CPU checker only: spica's Independent/MT19937 stream */
+};
+
+/* Render request: "sampleCount" and "maxDepth" as spica reads them, plus the
+ * counter-based sampler key. */
+typedef struct spk_render_desc {
+    int32_t spp;                 /* "sampleCount" */
+    int32_t max_depth;           /* "maxDepth" (spica default 16) */
+    int32_t sampler;             /* enum spk_sampler_type */
+    uint32_t seed;
+    int32_t rr_depth;            /* Russian roulette when bounces > rr_depth (spica: 3) */
+    int32_t first_sample;        /* sample index of the first pass (resume / split) */
+} spk_render_desc;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPK_TYPES_H_ */

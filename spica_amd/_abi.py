@@ -1,0 +1,178 @@
+"""ctypes mirror of include/spk_types.h and include/spk.h, and the loader of libspk.so.
+
+The library is loaded from the package directory only; a missing or stale build is
+an error (there is no CPU fallback for the render path).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libspk.so")
+
+SPK_MAT_DIFFUSE = 1
+SPK_MAT_CONDUCTOR = 2
+SPK_MAT_DIELECTRIC = 3
+SPK_MAT_ROUGHCONDUCTOR = 4
+
+SPK_SAMPLER_INDEPENDENT = 0
+SPK_SAMPLER_SOBOL = 1
+SPK_SAMPLER_MT19937 = 2
+
+SPK_OK = 0
+SPK_ERR_INVALID = 1
+SPK_ERR_HIP = 2
+SPK_ERR_NOMEM = 3
+SPK_ERR_STATE = 4
+
+c_f_p = ctypes.POINTER(ctypes.c_float)
+c_d_p = ctypes.POINTER(ctypes.c_double)
+c_i32_p = ctypes.POINTER(ctypes.c_int32)
+c_u32_p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class SpkMaterial(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("reflectance", ctypes.c_double * 3),
+        ("transmittance", ctypes.c_double * 3),
+        ("eta", ctypes.c_double * 3),
+        ("k", ctypes.c_double * 3),
+        ("int_ior", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+    ]
+
+
+class SpkSceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_tris", ctypes.c_int64),
+        ("verts", c_f_p),
+        ("normals", c_f_p),
+        ("mat_ids", c_i32_p),
+        ("light_ids", c_i32_p),
+        ("num_materials", ctypes.c_int32),
+        ("num_area_lights", ctypes.c_int32),
+        ("materials", ctypes.POINTER(SpkMaterial)),
+        ("area_tri", c_i32_p),
+        ("area_radiance", c_d_p),
+        ("env_width", ctypes.c_int32),
+        ("env_height", ctypes.c_int32),
+        ("env_rgb", c_f_p),
+        ("env_to_world", ctypes.c_double * 16),
+        ("env_scale", ctypes.c_double),
+        ("env_center", ctypes.c_double * 3),
+        ("env_radius", ctypes.c_double),
+    ]
+
+
+class SpkCameraDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("to_world", ctypes.c_double * 16),
+        ("fov_deg", ctypes.c_double),
+        ("aperture_radius", ctypes.c_double),
+        ("focus_distance", ctypes.c_double),
+    ]
+
+
+class SpkRenderDesc(ctypes.Structure):
+    _fields_ = [
+        ("spp", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("sampler", ctypes.c_int32),
+        ("seed", ctypes.c_uint32),
+        ("rr_depth", ctypes.c_int32),
+        ("first_sample", ctypes.c_int32),
+    ]
+
+
+class SpkSceneStats(ctypes.Structure):
+    _fields_ = [
+        ("num_tris", ctypes.c_int64),
+        ("num_nodes", ctypes.c_int64),
+        ("num_leaves", ctypes.c_int64),
+        ("max_depth", ctypes.c_int32),
+        ("num_lights", ctypes.c_int32),
+        ("build_ms", ctypes.c_double),
+        ("upload_ms", ctypes.c_double),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+class SpkRenderOpts(ctypes.Structure):
+    _fields_ = [
+        ("queue_size", ctypes.c_int64),
+        ("block_size", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("profile", ctypes.c_int32),
+    ]
+
+
+class SpkRenderStats(ctypes.Structure):
+    _fields_ = [
+        ("samples_done", ctypes.c_int64),
+        ("paths", ctypes.c_int64),
+        ("rays_extend", ctypes.c_int64),
+        ("rays_shadow", ctypes.c_int64),
+        ("rays_mis", ctypes.c_int64),
+        ("passes", ctypes.c_int64),
+        ("extend_ms", ctypes.c_double),
+        ("extend_launches", ctypes.c_int64),
+        ("extend_rays_timed", ctypes.c_int64),
+        ("shadow_ms", ctypes.c_double),
+        ("shadow_launches", ctypes.c_int64),
+        ("shade_ms", ctypes.c_double),
+    ]
+
+
+# name -> (restype, argtypes); every entry point declared in include/spk.h
+SIGNATURES = {
+    "spk_last_error": (ctypes.c_char_p, []),
+    "spk_version": (ctypes.c_char_p, []),
+    "spk_init": (ctypes.c_int, [ctypes.c_int]),
+    "spk_scene_create": (ctypes.c_int, [ctypes.POINTER(SpkSceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "spk_scene_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_scene_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkSceneStats)]),
+    "spk_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                 ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "spk_render_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkCameraDesc), ctypes.POINTER(SpkRenderDesc),
+                                         ctypes.POINTER(SpkRenderOpts), ctypes.POINTER(ctypes.c_void_p)]),
+    "spk_render_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_render_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "spk_render_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_render_film": (ctypes.c_int, [ctypes.c_void_p, c_d_p, c_d_p, c_d_p]),
+    "spk_render_image": (ctypes.c_int, [ctypes.c_void_p, c_f_p]),
+    "spk_render_relmse": (ctypes.c_int, [ctypes.c_void_p, c_d_p, ctypes.c_double, c_d_p]),
+    "spk_render_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkRenderStats)]),
+    "spk_sampler_u32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, c_u32_p, c_u32_p, c_u32_p,
+                                       c_u32_p, ctypes.c_int32]),
+}
+
+_lib = None
+
+
+class SpkError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libspk.so (built in-tree by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SpkError(f"{LIB_PATH} is missing: build the HIP extension first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != SPK_OK:
+        msg = load().spk_last_error().decode(errors="replace")
+        raise SpkError(f"{what} failed (status {rc}): {msg}")

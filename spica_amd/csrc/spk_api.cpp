@@ -1,0 +1,900 @@
+// spk_api.cpp -- C ABI of libspk.so: scene upload, batched traces, wavefront renders.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "spk_internal.h"
+#include "spk_sampler.h"
+
+namespace spk {
+
+static thread_local char g_err[1024] = "";
+static thread_local int g_device = 0;
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+int DevBuf::alloc(size_t nbytes) {
+    release();
+    if (nbytes == 0) nbytes = 16;
+    hipError_t e = hipMalloc(&p, nbytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", nbytes, hipGetErrorString(e));
+        return SPK_ERR_NOMEM;
+    }
+    bytes = nbytes;
+    return SPK_OK;
+}
+int DevBuf::upload(const void* src, size_t nbytes) {
+    int rc = alloc(nbytes);
+    if (rc) return rc;
+    if (src && nbytes) SPK_CHECK_HIP(hipMemcpy(p, src, nbytes, hipMemcpyHostToDevice));
+    return SPK_OK;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int trace_blocks_per_cu();
+int zero_u32(uint32_t* p, int n, hipStream_t st);
+int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
+                   uint32_t* out, hipStream_t st);
+
+}  // namespace spk
+
+using namespace spk;
+
+// ---------------------------------------------------------------------------------
+// double-precision helpers for the host-side set-up (spica computes in FP64)
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct D3 {
+    double x, y, z;
+};
+inline D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+inline D3 dsub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline D3 dadd(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline D3 dmul(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 dcross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline double dnorm(D3 a) { return sqrt(ddot(a, a)); }
+inline D3 dnormalize(D3 a) {
+    double n = dnorm(a);
+    return d3(a.x / n, a.y / n, a.z / n);
+}
+// vect_math.h:121-128
+inline void dcoord(D3 w, D3* u, D3* v) {
+    if (fabs(w.x) > fabs(w.y)) *u = dnormalize(d3(-w.z, 0.0, w.x));
+    else *u = dnormalize(d3(0.0, w.z, -w.y));
+    *v = dnormalize(dcross(w, *u));
+}
+inline float4 f4(D3 a, float w) { return make_float4((float)a.x, (float)a.y, (float)a.z, w); }
+inline double dgray(double r, double g, double b) { return 0.2126 * r + 0.7152 * g + 0.0722 * b; }
+
+// Triangle frame as spica builds it (triangle.cc:17-52, 119-138).
+struct TriFrame {
+    D3 p[3], n[3], fn, ng;
+    double area;
+    bool smooth;
+};
+TriFrame tri_frame(const float* v, const float* nrm) {
+    TriFrame t;
+    for (int i = 0; i < 3; i++) t.p[i] = d3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+    D3 e1 = dsub(t.p[1], t.p[0]), e2 = dsub(t.p[2], t.p[0]);
+    D3 c = dcross(e1, e2);
+    t.smooth = false;
+    if (!nrm) {
+        t.fn = dnormalize(c);
+        t.n[0] = t.n[1] = t.n[2] = t.fn;
+    } else {
+        for (int i = 0; i < 3; i++) t.n[i] = dnormalize(d3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]));
+        D3 fn = c;
+        if (dnorm(fn) < 1e-12) fn = dmul(dadd(dadd(t.n[0], t.n[1]), t.n[2]), 1.0 / 3.0);
+        t.fn = dnormalize(fn);
+        for (int i = 0; i < 3; i++)
+            if (fabs(ddot(t.n[i], t.fn)) < 1.0 - 1e-12) t.smooth = true;
+    }
+    D3 du, dv;
+    dcoord(t.fn, &du, &dv);
+    t.ng = dnormalize(dcross(du, dv));
+    t.area = 0.5 * dnorm(c);
+    return t;
+}
+
+// ---- environment map distribution (envmap.cc:17-47, mipmap.cc, sampling.cc) ----
+struct Mip {
+    int levels;
+    std::vector<int> w, h;
+    std::vector<std::vector<double>> px;
+    D3 texel(int l, int s, int t) const {
+        int W = w[l], H = h[l];
+        s = (s % W + W) % W;
+        t = (t % H + H) % H;
+        const double* p = &px[l][((size_t)t * W + s) * 3];
+        return d3(p[0], p[1], p[2]);
+    }
+    D3 bilinear(int l, double s0, double t0) const {
+        l = std::max(0, std::min(l, levels));
+        double s = s0 * w[l] - 0.5, t = t0 * h[l] - 0.5;
+        int si = (int)s, ti = (int)t;
+        double ds = s - si, dt = t - ti;
+        D3 r = dmul(texel(l, si, ti), (1 - ds) * (1 - dt));
+        r = dadd(r, dmul(texel(l, si + 1, ti), ds * (1 - dt)));
+        r = dadd(r, dmul(texel(l, si, ti + 1), (1 - ds) * dt));
+        r = dadd(r, dmul(texel(l, si + 1, ti + 1), ds * dt));
+        return r;
+    }
+    D3 lookup(double s, double t, double width) const {
+        double level = levels - 1 + log2(std::max(width, 1.0e-8));
+        if (level < 0) return bilinear(0, s, t);
+        if (level >= levels - 1) return texel(levels - 1, 0, 0);
+        int l = (int)level;
+        double delta = level - l;
+        return dadd(dmul(bilinear(l, s, t), 1 - delta), dmul(bilinear(l + 1, s, t), delta));
+    }
+};
+int round_up_pow2(int v) {
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+int log2_int(int v) {
+    int r = 0;
+    while (v) {
+        r++;
+        v >>= 1;
+    }
+    return r;
+}
+void build_mip(Mip& m, int w, int h, std::vector<double>&& base) {
+    int rx = w, ry = h;
+    bool p2 = (w && !(w & (w - 1))) && (h && !(h & (h - 1)));
+    if (!p2) {
+        rx = round_up_pow2(w);
+        ry = round_up_pow2(h);
+    }
+    m.levels = 1 + log2_int(std::max(rx, ry));
+    m.w.assign(m.levels, 0);
+    m.h.assign(m.levels, 0);
+    m.px.assign(m.levels, {});
+    m.w[0] = w;
+    m.h[0] = h;
+    m.px[0] = std::move(base);
+    for (int i = 1; i < m.levels; i++) {
+        int sr = std::max(1, m.w[i - 1] / 2), tr = std::max(1, m.h[i - 1] / 2);
+        m.w[i] = sr;
+        m.h[i] = tr;
+        m.px[i].assign((size_t)sr * tr * 3, 0.0);
+        for (int y = 0; y < tr; y++)
+            for (int x = 0; x < sr; x++) {
+                D3 s = dadd(dadd(m.texel(i - 1, 2 * x, 2 * y), m.texel(i - 1, 2 * x + 1, 2 * y)),
+                            dadd(m.texel(i - 1, 2 * x, 2 * y + 1), m.texel(i - 1, 2 * x + 1, 2 * y + 1)));
+                s = dmul(s, 0.25);
+                double* p = &m.px[i][((size_t)y * sr + x) * 3];
+                p[0] = s.x;
+                p[1] = s.y;
+                p[2] = s.z;
+            }
+    }
+}
+struct Dist1 {
+    std::vector<double> func, cdf;
+    double integral;
+    void init(const double* f, int n) {
+        func.assign(f, f + n);
+        cdf.assign(n + 1, 0.0);
+        for (int i = 1; i <= n; i++) cdf[i] = cdf[i - 1] + func[i - 1] / n;
+        integral = cdf[n];
+        if (integral == 0.0) {
+            for (int i = 1; i <= n; i++) cdf[i] = (double)i / n;
+        } else {
+            for (int i = 1; i <= n; i++) cdf[i] /= integral;
+        }
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------------
+struct spk_scene {
+    int device = 0;
+    SceneDev dev;
+    DevBuf nodes, tris, tri_ng, tri_ids, tri_n, mats, lights;
+    DevBuf env_tex, env_marg_cdf, env_marg_pdf, env_cond_cdf, env_cond_pdf, env_pdf2d;
+    DevBuf trace_work, trace_spill;
+    int trace_grid = 0;
+    int spill_cap = 0;
+    spk_scene_stats stats;
+};
+
+struct spk_render {
+    spk_scene* scene = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ConfigDev cfg;
+    spk_render_desc desc;
+    spk_render_opts opts;
+    RenderBufs bufs;
+    LaunchShape shape;
+    DevBuf qa_o[2], qa_d[2], qa_b[2], hit, qs_o, qs_d, qs_c, qm_o, qm_d, qm_c, path_L;
+    DevBuf film_sum, film_c, film_sq, film_sqc, spill, cfg_dev, ps_dev, scratch;
+    uint32_t capacity = 0;
+    int64_t samples_done = 0;
+    double extend_ms = 0.0;
+    int64_t extend_launches = 0;
+    int64_t extend_rays_timed = 0;
+    std::vector<hipEvent_t> events;
+};
+
+extern "C" {
+
+const char* spk_last_error(void) { return g_err; }
+
+const char* spk_version(void) { return "spica-amd 0.1 (gfx950 wavefront path tracer)"; }
+
+int spk_init(int device) {
+    clear_error();
+    int n = 0;
+    SPK_CHECK_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_error("spk_init: device %d out of range (%d devices)", device, n);
+        return SPK_ERR_INVALID;
+    }
+    SPK_CHECK_HIP(hipSetDevice(device));
+    g_device = device;
+    return SPK_OK;
+}
+
+int spk_scene_create(const spk_scene_desc* d, spk_scene** out) {
+    clear_error();
+    if (!d || !out) {
+        set_error("spk_scene_create: null argument");
+        return SPK_ERR_INVALID;
+    }
+    *out = nullptr;
+    const int64_t n = d->num_tris;
+    if (n < 0 || (n > 0 && (!d->verts || !d->mat_ids)) || n >= (int64_t(1) << 28)) {
+        set_error("spk_scene_create: bad triangle arrays (num_tris=%lld)", (long long)n);
+        return SPK_ERR_INVALID;
+    }
+    if (d->num_materials <= 0 && n > 0) {
+        set_error("spk_scene_create: no materials");
+        return SPK_ERR_INVALID;
+    }
+    if (d->num_area_lights < 0 || (d->num_area_lights > 0 && (!d->area_tri || !d->area_radiance || !d->light_ids))) {
+        set_error("spk_scene_create: bad area light arrays");
+        return SPK_ERR_INVALID;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        if (d->mat_ids[i] < 0 || d->mat_ids[i] >= d->num_materials) {
+            set_error("spk_scene_create: triangle %lld has material %d outside [0,%d)", (long long)i, d->mat_ids[i],
+                      d->num_materials);
+            return SPK_ERR_INVALID;
+        }
+        int li = d->light_ids ? d->light_ids[i] : -1;
+        if (li < -1 || li >= d->num_area_lights || (li >= 0 && d->area_tri[li] != i)) {
+            set_error("spk_scene_create: triangle %lld has inconsistent light id %d", (long long)i, li);
+            return SPK_ERR_INVALID;
+        }
+    }
+    for (int i = 0; i < d->num_area_lights; i++) {
+        int t = d->area_tri[i];
+        if (t < 0 || t >= n || d->light_ids[t] != i) {
+            set_error("spk_scene_create: area light %d names triangle %d inconsistently", i, t);
+            return SPK_ERR_INVALID;
+        }
+    }
+    for (int m = 0; m < d->num_materials; m++) {
+        int ty = d->materials[m].type;
+        if (ty < SPK_MAT_DIFFUSE || ty > SPK_MAT_ROUGHCONDUCTOR) {
+            set_error("spk_scene_create: material %d has unknown type %d", m, ty);
+            return SPK_ERR_INVALID;
+        }
+    }
+    if (d->env_width > 0 && (d->env_height <= 0 || !d->env_rgb)) {
+        set_error("spk_scene_create: bad environment map");
+        return SPK_ERR_INVALID;
+    }
+
+    std::unique_ptr<spk_scene> s(new spk_scene());
+    s->device = g_device;
+    SPK_CHECK_HIP(hipSetDevice(s->device));
+    memset(&s->dev, 0, sizeof(s->dev));
+    memset(&s->stats, 0, sizeof(s->stats));
+    int rc;
+
+    // BVH
+    auto t0 = std::chrono::steady_clock::now();
+    BvhBuildOpts bo;
+    BvhResult bvh = build_bvh(d->verts, n, bo);
+    auto t1 = std::chrono::steady_clock::now();
+
+    // Leaf-ordered triangle data (FP64 set-up, FP32 storage).
+    std::vector<float4> tris((size_t)n * 3), tri_ng((size_t)n), tri_n;
+    std::vector<int2> tri_ids((size_t)n);
+    bool any_smooth = false;
+    std::vector<TriFrame> frames((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        frames[(size_t)i] = tri_frame(d->verts + 9 * i, d->normals ? d->normals + 9 * i : nullptr);
+        any_smooth = any_smooth || frames[(size_t)i].smooth;
+    }
+    if (any_smooth) tri_n.resize((size_t)n * 3);
+    for (int64_t k = 0; k < n; k++) {
+        uint32_t prim = bvh.order[(size_t)k];
+        const TriFrame& f = frames[prim];
+        const float* v = d->verts + 9 * (size_t)prim;
+        float4 T0 = make_float4(v[0], v[1], v[2], 0.0f);
+        memcpy(&T0.w, &prim, 4);
+        tris[3 * k] = T0;
+        tris[3 * k + 1] = f4(dsub(f.p[1], f.p[0]), 0.0f);
+        tris[3 * k + 2] = f4(dsub(f.p[2], f.p[0]), 0.0f);
+        uint32_t mbits = (uint32_t)d->mat_ids[prim] | (f.smooth ? 0x80000000u : 0u);
+        float w;
+        memcpy(&w, &mbits, 4);
+        tri_ng[k] = f4(f.ng, w);
+        tri_ids[k] = make_int2((int)prim, d->light_ids ? d->light_ids[prim] : -1);
+        if (any_smooth)
+            for (int j = 0; j < 3; j++) tri_n[3 * k + j] = f4(f.n[j], 0.0f);
+    }
+
+    // Materials (bsdfs/*.cc setScatterFuncs).
+    std::vector<MatDev> mats((size_t)std::max(1, d->num_materials));
+    for (int m = 0; m < d->num_materials; m++) {
+        const spk_material& M = d->materials[m];
+        MatDev md;
+        memset(&md, 0, sizeof(md));
+        auto cl = [](double x) { return std::max(0.0, std::min(x, 1.0e32)); };
+        int lobe = LOBE_NONE;
+        D3 R = d3(1, 1, 1), T = d3(0, 0, 0);
+        double etaB = 1.0, alpha = 0.0;
+        switch (M.type) {
+        case SPK_MAT_DIFFUSE:
+            R = d3(cl(M.reflectance[0]), cl(M.reflectance[1]), cl(M.reflectance[2]));
+            lobe = (R.x == 0 && R.y == 0 && R.z == 0) ? LOBE_NONE : LOBE_LAMBERT;
+            break;
+        case SPK_MAT_CONDUCTOR:
+            lobe = LOBE_SPECREFL;
+            break;
+        case SPK_MAT_DIELECTRIC:
+            R = d3(cl(M.reflectance[0]), cl(M.reflectance[1]), cl(M.reflectance[2]));
+            T = d3(cl(M.transmittance[0]), cl(M.transmittance[1]), cl(M.transmittance[2]));
+            etaB = dgray(M.int_ior, M.int_ior, M.int_ior);
+            lobe = (R.x == 0 && R.y == 0 && R.z == 0 && T.x == 0 && T.y == 0 && T.z == 0) ? LOBE_NONE : LOBE_FRESNELSPEC;
+            break;
+        case SPK_MAT_ROUGHCONDUCTOR:
+            alpha = dgray(M.alpha, M.alpha, M.alpha);
+            lobe = alpha == 0.0 ? LOBE_SPECREFL : LOBE_MICROREFL;
+            break;
+        }
+        float lw;
+        memcpy(&lw, &lobe, 4);
+        md.a = f4(R, lw);
+        md.b = f4(T, (float)etaB);
+        md.c = f4(d3(M.eta[0], M.eta[1], M.eta[2]), (float)alpha);
+        md.d = f4(d3(M.k[0], M.k[1], M.k[2]), 0.0f);
+        mats[(size_t)m] = md;
+    }
+
+    // Area lights.
+    std::vector<LightDev> lights((size_t)std::max(1, d->num_area_lights));
+    for (int i = 0; i < d->num_area_lights; i++) {
+        const TriFrame& f = frames[(size_t)d->area_tri[i]];
+        LightDev L;
+        L.p0 = f4(f.p[0], (float)f.area);
+        L.e1 = f4(dsub(f.p[1], f.p[0]), 0.0f);
+        L.e2 = f4(dsub(f.p[2], f.p[0]), 0.0f);
+        L.n0 = f4(f.n[0], 0.0f);
+        L.n1 = f4(f.n[1], 0.0f);
+        L.n2 = f4(f.n[2], 0.0f);
+        L.L = f4(d3(d->area_radiance[3 * i], d->area_radiance[3 * i + 1], d->area_radiance[3 * i + 2]), 0.0f);
+        L.ng = f4(f.ng, 0.0f);
+        lights[(size_t)i] = L;
+    }
+
+    // Environment map.
+    EnvDev env;
+    memset(&env, 0, sizeof(env));
+    if (d->env_width > 0) {
+        int w = d->env_width, h = d->env_height;
+        std::vector<double> base((size_t)w * h * 3);
+        for (size_t i = 0; i < base.size(); i++) base[i] = (double)d->env_rgb[i] * d->env_scale;
+        Mip mip;
+        build_mip(mip, w, h, std::move(base));
+        double filter = 1.0 / std::max(w, h);
+        std::vector<double> g((size_t)w * h);
+        const double PI = 3.14159265358979323846;
+        for (int v = 0; v < h; v++) {
+            double vp = (double)(v + 0.5) / h, st = sin(PI * (v + 0.5) / h);
+            for (int u = 0; u < w; u++) {
+                D3 c = mip.lookup((double)(u + 0.5) / w, vp, filter);
+                g[(size_t)v * w + u] = dgray(c.x, c.y, c.z) * st;
+            }
+        }
+        std::vector<Dist1> cond((size_t)h);
+        std::vector<double> mf((size_t)h);
+        for (int v = 0; v < h; v++) {
+            cond[(size_t)v].init(&g[(size_t)v * w], w);
+            mf[(size_t)v] = cond[(size_t)v].integral;
+        }
+        Dist1 marg;
+        marg.init(mf.data(), h);
+        std::vector<float4> tex((size_t)w * h);
+        for (size_t i = 0; i < tex.size(); i++)
+            tex[i] = make_float4((float)mip.px[0][3 * i], (float)mip.px[0][3 * i + 1], (float)mip.px[0][3 * i + 2], 0.0f);
+        std::vector<float> mcdf((size_t)h + 1), mpdf((size_t)h), ccdf((size_t)h * (w + 1)), cpdf((size_t)h * w),
+            pdf2((size_t)h * w);
+        for (int v = 0; v <= h; v++) mcdf[(size_t)v] = (float)marg.cdf[(size_t)v];
+        for (int v = 0; v < h; v++) {
+            mpdf[(size_t)v] = (float)(marg.func[(size_t)v] / marg.integral);
+            for (int u = 0; u <= w; u++) ccdf[(size_t)v * (w + 1) + u] = (float)cond[(size_t)v].cdf[(size_t)u];
+            for (int u = 0; u < w; u++) {
+                cpdf[(size_t)v * w + u] = (float)(cond[(size_t)v].func[(size_t)u] / cond[(size_t)v].integral);
+                pdf2[(size_t)v * w + u] = (float)(cond[(size_t)v].func[(size_t)u] / marg.integral);
+            }
+        }
+        if ((rc = s->env_tex.upload(tex.data(), tex.size() * sizeof(float4)))) return rc;
+        if ((rc = s->env_marg_cdf.upload(mcdf.data(), mcdf.size() * 4))) return rc;
+        if ((rc = s->env_marg_pdf.upload(mpdf.data(), mpdf.size() * 4))) return rc;
+        if ((rc = s->env_cond_cdf.upload(ccdf.data(), ccdf.size() * 4))) return rc;
+        if ((rc = s->env_cond_pdf.upload(cpdf.data(), cpdf.size() * 4))) return rc;
+        if ((rc = s->env_pdf2d.upload(pdf2.data(), pdf2.size() * 4))) return rc;
+        env.present = 1;
+        env.w = w;
+        env.h = h;
+        env.radius = (float)d->env_radius;
+        env.tex = s->env_tex.as<float4>();
+        env.marg_cdf = s->env_marg_cdf.as<float>();
+        env.marg_pdf = s->env_marg_pdf.as<float>();
+        env.cond_cdf = s->env_cond_cdf.as<float>();
+        env.cond_pdf = s->env_cond_pdf.as<float>();
+        env.pdf2d = s->env_pdf2d.as<float>();
+        // envmap.cc:19 keeps the transpose of toWorld; Light keeps its inverse.
+        double l2w[3][3], w2l[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) l2w[i][j] = d->env_to_world[j * 4 + i];
+        double det = l2w[0][0] * (l2w[1][1] * l2w[2][2] - l2w[1][2] * l2w[2][1]) -
+                     l2w[0][1] * (l2w[1][0] * l2w[2][2] - l2w[1][2] * l2w[2][0]) +
+                     l2w[0][2] * (l2w[1][0] * l2w[2][1] - l2w[1][1] * l2w[2][0]);
+        if (det == 0.0) {
+            set_error("spk_scene_create: singular environment transform");
+            return SPK_ERR_INVALID;
+        }
+        double id = 1.0 / det;
+        w2l[0][0] = (l2w[1][1] * l2w[2][2] - l2w[1][2] * l2w[2][1]) * id;
+        w2l[0][1] = (l2w[0][2] * l2w[2][1] - l2w[0][1] * l2w[2][2]) * id;
+        w2l[0][2] = (l2w[0][1] * l2w[1][2] - l2w[0][2] * l2w[1][1]) * id;
+        w2l[1][0] = (l2w[1][2] * l2w[2][0] - l2w[1][0] * l2w[2][2]) * id;
+        w2l[1][1] = (l2w[0][0] * l2w[2][2] - l2w[0][2] * l2w[2][0]) * id;
+        w2l[1][2] = (l2w[0][2] * l2w[1][0] - l2w[0][0] * l2w[1][2]) * id;
+        w2l[2][0] = (l2w[1][0] * l2w[2][1] - l2w[1][1] * l2w[2][0]) * id;
+        w2l[2][1] = (l2w[0][1] * l2w[2][0] - l2w[0][0] * l2w[2][1]) * id;
+        w2l[2][2] = (l2w[0][0] * l2w[1][1] - l2w[0][1] * l2w[1][0]) * id;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                env.l2w[i * 3 + j] = (float)l2w[i][j];
+                env.w2l[i * 3 + j] = (float)w2l[i][j];
+            }
+    }
+
+    // Upload.
+    std::vector<BvhNodeHost>& nodes = bvh.nodes;
+    if ((rc = s->nodes.upload(nodes.data(), nodes.size() * sizeof(BvhNodeHost)))) return rc;
+    if ((rc = s->tris.upload(tris.data(), tris.size() * sizeof(float4)))) return rc;
+    if ((rc = s->tri_ng.upload(tri_ng.data(), tri_ng.size() * sizeof(float4)))) return rc;
+    if ((rc = s->tri_ids.upload(tri_ids.data(), tri_ids.size() * sizeof(int2)))) return rc;
+    if (any_smooth && (rc = s->tri_n.upload(tri_n.data(), tri_n.size() * sizeof(float4)))) return rc;
+    if ((rc = s->mats.upload(mats.data(), mats.size() * sizeof(MatDev)))) return rc;
+    if ((rc = s->lights.upload(lights.data(), lights.size() * sizeof(LightDev)))) return rc;
+    auto t2 = std::chrono::steady_clock::now();
+
+    SceneDev& sd = s->dev;
+    sd.nodes = s->nodes.as<float4>();
+    sd.tris = s->tris.as<float4>();
+    sd.tri_ng = s->tri_ng.as<float4>();
+    sd.tri_ids = s->tri_ids.as<int2>();
+    sd.tri_n = any_smooth ? s->tri_n.as<float4>() : nullptr;
+    sd.mats = s->mats.as<MatDev>();
+    sd.lights = s->lights.as<LightDev>();
+    sd.num_areas = d->num_area_lights;
+    sd.num_lights = d->num_area_lights + (env.present ? 1 : 0);
+    sd.has_bvh = n > 0 ? 1 : 0;
+    sd.env = env;
+
+    s->spill_cap = std::max(8, bvh.max_depth + 2);
+    s->stats.num_tris = n;
+    s->stats.num_nodes = (int64_t)nodes.size();
+    s->stats.num_leaves = bvh.num_leaves;
+    s->stats.max_depth = bvh.max_depth;
+    s->stats.num_lights = sd.num_lights;
+    s->stats.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->stats.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    s->stats.device_bytes = (int64_t)(s->nodes.bytes + s->tris.bytes + s->tri_ng.bytes + s->tri_ids.bytes +
+                                      s->tri_n.bytes + s->mats.bytes + s->lights.bytes + s->env_tex.bytes +
+                                      s->env_cond_cdf.bytes + s->env_cond_pdf.bytes + s->env_pdf2d.bytes);
+    *out = s.release();
+    return SPK_OK;
+}
+
+int spk_scene_destroy(spk_scene* s) {
+    clear_error();
+    if (!s) return SPK_OK;
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
+    DevBuf* bufs[] = {&s->nodes, &s->tris, &s->tri_ng, &s->tri_ids, &s->tri_n, &s->mats, &s->lights,
+                      &s->env_tex, &s->env_marg_cdf, &s->env_marg_pdf, &s->env_cond_cdf, &s->env_cond_pdf,
+                      &s->env_pdf2d, &s->trace_work, &s->trace_spill};
+    for (DevBuf* b : bufs) b->release();
+    delete s;
+    return SPK_OK;
+}
+
+int spk_scene_stats_get(const spk_scene* s, spk_scene_stats* out) {
+    clear_error();
+    if (!s || !out) {
+        set_error("spk_scene_stats_get: null argument");
+        return SPK_ERR_INVALID;
+    }
+    *out = s->stats;
+    return SPK_OK;
+}
+
+int spk_trace(spk_scene* s, const float* org, const float* dir, const float* tmax, int64_t n, int32_t any_hit,
+              int32_t* prim_out, float* t_out, void* stream) {
+    clear_error();
+    if (!s || n < 0 || (n > 0 && (!org || !dir || !prim_out))) {
+        set_error("spk_trace: bad arguments");
+        return SPK_ERR_INVALID;
+    }
+    if (n >= (int64_t(1) << 32) - 64) {
+        set_error("spk_trace: n too large");
+        return SPK_ERR_INVALID;
+    }
+    if (n == 0) return SPK_OK;
+    SPK_CHECK_HIP(hipSetDevice(s->device));
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (!s->trace_work.p) {
+        hipDeviceProp_t prop;
+        SPK_CHECK_HIP(hipGetDeviceProperties(&prop, s->device));
+        s->trace_grid = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu());
+        if ((rc = s->trace_work.alloc(64))) return rc;
+        if ((rc = s->trace_spill.alloc((size_t)s->trace_grid * 128 * s->spill_cap * 4))) return rc;
+    }
+    if ((rc = zero_u32(s->trace_work.as<uint32_t>(), 16, st))) return rc;
+    TraceBufs b;
+    b.org = org;
+    b.dir = dir;
+    b.tmax = tmax;
+    b.prim_out = prim_out;
+    b.t_out = t_out;
+    b.n = n;
+    b.any_hit = any_hit ? 1 : 0;
+    return launch_trace(s->dev, b, s->trace_work.as<uint32_t>(), s->trace_spill.as<uint32_t>(), s->spill_cap,
+                        s->trace_grid, st);
+}
+
+int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render_desc* desc, const spk_render_opts* opts,
+                      spk_render** out) {
+    clear_error();
+    if (!s || !cam || !desc || !out) {
+        set_error("spk_render_create: null argument");
+        return SPK_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height >= (int64_t(1) << 31)) {
+        set_error("spk_render_create: bad film size %dx%d", cam->width, cam->height);
+        return SPK_ERR_INVALID;
+    }
+    if (desc->max_depth < 0 || desc->max_depth > 60) {
+        set_error("spk_render_create: maxDepth %d outside [0, 60]", desc->max_depth);
+        return SPK_ERR_INVALID;
+    }
+    if (desc->sampler != SPK_SAMPLER_INDEPENDENT && desc->sampler != SPK_SAMPLER_SOBOL) {
+        set_error("spk_render_create: sampler %d not available on the GPU", desc->sampler);
+        return SPK_ERR_INVALID;
+    }
+    if (cam->to_world[12] != 0.0 || cam->to_world[13] != 0.0 || cam->to_world[14] != 0.0 || cam->to_world[15] != 1.0) {
+        set_error("spk_render_create: camera toWorld must be affine");
+        return SPK_ERR_INVALID;
+    }
+    std::unique_ptr<spk_render> r(new spk_render());
+    r->scene = s;
+    r->device = s->device;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    r->desc = *desc;
+    spk_render_opts o;
+    memset(&o, 0, sizeof(o));
+    if (opts) o = *opts;
+    if (o.queue_size <= 0) o.queue_size = int64_t(1) << 21;
+    if (o.queue_size < 64) o.queue_size = 64;
+    if (o.queue_size > (int64_t(1) << 24)) o.queue_size = int64_t(1) << 24;
+    r->opts = o;
+
+    ConfigDev& c = r->cfg;
+    memset(&c, 0, sizeof(c));
+    const double PI = 3.14159265358979323846;
+    double tanHalf = tan(cam->fov_deg * PI / 180.0 / 2.0);
+    double aspect = (double)cam->width / cam->height;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) c.cam.m[i * 4 + j] = (float)cam->to_world[i * 4 + j];
+    c.cam.sx_scale = (float)(aspect * tanHalf);
+    c.cam.sy_scale = (float)tanHalf;
+    c.cam.lens_r = (float)cam->aperture_radius;
+    c.cam.focal = (float)cam->focus_distance;
+    c.cam.W = cam->width;
+    c.cam.H = cam->height;
+    c.W = cam->width;
+    c.H = cam->height;
+    c.npix = (uint32_t)cam->width * (uint32_t)cam->height;
+    uint32_t qcap = (uint32_t)o.queue_size;
+    if (c.npix <= qcap) {
+        c.kmax = qcap / c.npix;
+        c.chunks = 1;
+        r->capacity = c.kmax * c.npix;
+    } else {
+        c.kmax = 1;
+        c.chunks = (c.npix + qcap - 1) / qcap;
+        r->capacity = qcap;
+    }
+    c.queue = c.chunks > 1 ? qcap : r->capacity;
+    c.max_depth = desc->max_depth;
+    c.rr_depth = desc->rr_depth;
+    c.sampler = desc->sampler;
+    c.seed = desc->seed;
+
+    // Buffers.
+    int rc;
+    size_t Q = r->capacity;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = r->qa_o[k].alloc(Q * 16)) || (rc = r->qa_d[k].alloc(Q * 16)) || (rc = r->qa_b[k].alloc(Q * 16))) return rc;
+    }
+    if ((rc = r->hit.alloc(Q * 16)) || (rc = r->qs_o.alloc(Q * 16)) || (rc = r->qs_d.alloc(Q * 16)) ||
+        (rc = r->qs_c.alloc(Q * 16)) || (rc = r->qm_o.alloc(Q * 16)) || (rc = r->qm_d.alloc(Q * 16)) ||
+        (rc = r->qm_c.alloc(Q * 16)) || (rc = r->path_L.alloc(Q * 16)))
+        return rc;
+    size_t P = c.npix;
+    if ((rc = r->film_sum.alloc(P * 16)) || (rc = r->film_c.alloc(P * 16)) || (rc = r->film_sq.alloc(P * 16)) ||
+        (rc = r->film_sqc.alloc(P * 16)))
+        return rc;
+    hipDeviceProp_t prop;
+    SPK_CHECK_HIP(hipGetDeviceProperties(&prop, r->device));
+    r->shape.block = 128;
+    r->shape.grid_trace = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu());
+    r->shape.grid_stream = prop.multiProcessorCount * 8;
+    if ((rc = r->spill.alloc((size_t)r->shape.grid_trace * 128 * s->spill_cap * 4))) return rc;
+    if ((rc = r->cfg_dev.alloc(sizeof(ConfigDev))) || (rc = r->ps_dev.alloc(sizeof(PassState)))) return rc;
+    if ((rc = r->scratch.alloc(4096 * sizeof(double)))) return rc;
+    SPK_CHECK_HIP(hipMemset(r->ps_dev.p, 0, sizeof(PassState)));
+    SPK_CHECK_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+
+    RenderBufs& b = r->bufs;
+    for (int k = 0; k < 2; k++) {
+        b.qa_o[k] = r->qa_o[k].as<float4>();
+        b.qa_d[k] = r->qa_d[k].as<float4>();
+        b.qa_b[k] = r->qa_b[k].as<float4>();
+    }
+    b.hit = r->hit.as<float4>();
+    b.qs_o = r->qs_o.as<float4>();
+    b.qs_d = r->qs_d.as<float4>();
+    b.qs_c = r->qs_c.as<float4>();
+    b.qm_o = r->qm_o.as<float4>();
+    b.qm_d = r->qm_d.as<float4>();
+    b.qm_c = r->qm_c.as<float4>();
+    b.path_L = r->path_L.as<float4>();
+    b.film_sum = r->film_sum.as<float4>();
+    b.film_c = r->film_c.as<float4>();
+    b.film_sq = r->film_sq.as<float4>();
+    b.film_sqc = r->film_sqc.as<float4>();
+    b.spill = r->spill.as<uint32_t>();
+    b.spill_cap = s->spill_cap;
+    b.cfg = r->cfg_dev.as<ConfigDev>();
+    b.ps = r->ps_dev.as<PassState>();
+    if ((rc = launch_clear_film(b, c.npix, r->stream))) return rc;
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    *out = r.release();
+    return SPK_OK;
+}
+
+int spk_render_destroy(spk_render* r) {
+    clear_error();
+    if (!r) return SPK_OK;
+    (void)hipSetDevice(r->device);
+    if (r->stream) {
+        (void)hipStreamSynchronize(r->stream);
+        (void)hipStreamDestroy(r->stream);
+    }
+    for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    DevBuf* bufs[] = {&r->qa_o[0], &r->qa_o[1], &r->qa_d[0], &r->qa_d[1], &r->qa_b[0], &r->qa_b[1], &r->hit,
+                      &r->qs_o, &r->qs_d, &r->qs_c, &r->qm_o, &r->qm_d, &r->qm_c, &r->path_L, &r->film_sum,
+                      &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch};
+    for (DevBuf* b : bufs) b->release();
+    delete r;
+    return SPK_OK;
+}
+
+int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
+    clear_error();
+    if (!r || n_spp < 0) {
+        set_error("spk_render_run: bad arguments");
+        return SPK_ERR_INVALID;
+    }
+    if (n_spp == 0) return SPK_OK;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    ConfigDev& c = r->cfg;
+    int64_t first = (int64_t)r->desc.first_sample + r->samples_done;
+    if (first + n_spp >= (int64_t(1) << 31)) {
+        set_error("spk_render_run: sample index overflow");
+        return SPK_ERR_INVALID;
+    }
+    c.first_sample = (uint32_t)first;
+    c.end_sample = (uint32_t)(first + n_spp);
+    SPK_CHECK_HIP(hipMemcpyAsync(r->cfg_dev.p, &c, sizeof(ConfigDev), hipMemcpyHostToDevice, r->stream));
+    int rc;
+    if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
+    int64_t passes = c.chunks > 1 ? (int64_t)n_spp * c.chunks : ((int64_t)n_spp + c.kmax - 1) / c.kmax;
+    int nev = 2 * (c.max_depth + 1);
+    if (r->opts.profile) {
+        size_t need = (size_t)passes * nev;
+        while (r->events.size() < need) {
+            hipEvent_t e;
+            SPK_CHECK_HIP(hipEventCreate(&e));
+            r->events.push_back(e);
+        }
+    }
+    for (int64_t p = 0; p < passes; p++) {
+        hipEvent_t* ev = r->opts.profile ? &r->events[(size_t)p * nev] : nullptr;
+        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0))) return rc;
+    }
+    r->samples_done += n_spp;
+    if (r->opts.profile) {
+        SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+        for (int64_t p = 0; p < passes; p++)
+            for (int d = 0; d <= c.max_depth; d++) {
+                float ms = 0.0f;
+                SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[(size_t)p * nev + 2 * d], r->events[(size_t)p * nev + 2 * d + 1]));
+                r->extend_ms += ms;
+                r->extend_launches++;
+            }
+    }
+    if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    return SPK_OK;
+}
+
+int spk_render_sync(spk_render* r) {
+    clear_error();
+    if (!r) return SPK_ERR_INVALID;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    return SPK_OK;
+}
+
+int spk_render_film(spk_render* r, double* sum_rgb, double* sumsq_rgb, double* count) {
+    clear_error();
+    if (!r) return SPK_ERR_INVALID;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    size_t P = r->cfg.npix;
+    std::vector<float4> s(P), c(P), q(P), qc(P);
+    SPK_CHECK_HIP(hipMemcpy(s.data(), r->film_sum.p, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(c.data(), r->film_c.p, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(q.data(), r->film_sq.p, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(qc.data(), r->film_sqc.p, P * 16, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; i++) {
+        // The Kahan compensation holds the negated lost low-order part.
+        if (sum_rgb) {
+            sum_rgb[3 * i] = (double)s[i].x - (double)c[i].x;
+            sum_rgb[3 * i + 1] = (double)s[i].y - (double)c[i].y;
+            sum_rgb[3 * i + 2] = (double)s[i].z - (double)c[i].z;
+        }
+        if (sumsq_rgb) {
+            sumsq_rgb[3 * i] = (double)q[i].x - (double)qc[i].x;
+            sumsq_rgb[3 * i + 1] = (double)q[i].y - (double)qc[i].y;
+            sumsq_rgb[3 * i + 2] = (double)q[i].z - (double)qc[i].z;
+        }
+        if (count) count[i] = (double)s[i].w;
+    }
+    return SPK_OK;
+}
+
+int spk_render_image(spk_render* r, float* rgb) {
+    clear_error();
+    if (!r || !rgb) return SPK_ERR_INVALID;
+    size_t P = r->cfg.npix;
+    std::vector<double> sum(P * 3), cnt(P);
+    int rc = spk_render_film(r, sum.data(), nullptr, cnt.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < P; i++)
+        for (int k = 0; k < 3; k++) rgb[3 * i + k] = (float)(sum[3 * i + k] / (cnt[i] + 1e-12));
+    return SPK_OK;
+}
+
+int spk_render_relmse(spk_render* r, const double* ref_rgb, double eps, double* out) {
+    clear_error();
+    if (!r || !ref_rgb || !out) return SPK_ERR_INVALID;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    size_t P = r->cfg.npix;
+    DevBuf ref;
+    int rc;
+    if ((rc = ref.upload(ref_rgb, P * 3 * sizeof(double)))) return rc;
+    const int nblocks = 1024;
+    if ((rc = launch_relmse(r->bufs, r->cfg, ref.as<double>(), eps, r->scratch.as<double>(), nblocks, r->stream))) {
+        ref.release();
+        return rc;
+    }
+    double total = 0.0;
+    SPK_CHECK_HIP(hipMemcpyAsync(&total, r->scratch.as<double>() + nblocks, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    ref.release();
+    *out = total / (double)(P * 3);
+    return SPK_OK;
+}
+
+int spk_render_stats_get(spk_render* r, spk_render_stats* out) {
+    clear_error();
+    if (!r || !out) return SPK_ERR_INVALID;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    PassState ps;
+    SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->samples_done = r->samples_done;
+    out->paths = (int64_t)ps.total_paths;
+    out->rays_extend = (int64_t)ps.total_extend;
+    out->rays_shadow = (int64_t)ps.total_shadow;
+    out->rays_mis = (int64_t)ps.total_mis;
+    out->passes = (int64_t)ps.total_samples;
+    out->extend_ms = r->extend_ms;
+    out->extend_launches = r->extend_launches;
+    return SPK_OK;
+}
+
+int spk_sampler_u32(int32_t kind, uint32_t seed, int64_t n, const uint32_t* pixel, const uint32_t* sample,
+                    const uint32_t* dim, uint32_t* out, int32_t on_device) {
+    clear_error();
+    if (n < 0 || (n > 0 && (!pixel || !sample || !dim || !out)) || (kind != SAMPLER_INDEPENDENT && kind != SAMPLER_SOBOL)) {
+        set_error("spk_sampler_u32: bad arguments");
+        return SPK_ERR_INVALID;
+    }
+    if (!on_device) {
+        for (int64_t i = 0; i < n; i++) out[i] = sample_u32(kind, seed, pixel[i], sample[i], dim[i]);
+        return SPK_OK;
+    }
+    if (n == 0) return SPK_OK;
+    SPK_CHECK_HIP(hipSetDevice(g_device));
+    DevBuf a, b, c, o;
+    int rc;
+    if ((rc = a.upload(pixel, n * 4)) || (rc = b.upload(sample, n * 4)) || (rc = c.upload(dim, n * 4)) || (rc = o.alloc(n * 4)))
+        return rc;
+    if ((rc = launch_sampler(kind, seed, n, a.as<uint32_t>(), b.as<uint32_t>(), c.as<uint32_t>(), o.as<uint32_t>(), nullptr)))
+        return rc;
+    SPK_CHECK_HIP(hipDeviceSynchronize());
+    SPK_CHECK_HIP(hipMemcpy(out, o.p, n * 4, hipMemcpyDeviceToHost));
+    a.release();
+    b.release();
+    c.release();
+    o.release();
+    return SPK_OK;
+}
+
+}  // extern "C"

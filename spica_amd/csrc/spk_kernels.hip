@@ -1,0 +1,1272 @@
+// spk_kernels.hip -- gfx950 kernels of the wavefront path tracer.
+//
+// spica traces one path per thread-loop iteration, depth first, on the CPU
+// (sources/integrators/path/path.cc:42-125, sources/core/integrator.cc:77-97).
+// Here the path's bounces are split into stages over compacted SoA queues:
+//
+//   pass_init -> camera -> [extend -> shade -> shadow -> mis] x (maxDepth+1) -> film
+//
+// * extend / shadow / mis: persistent BVH traversal (closest hit / any hit /
+//   closest hit), rays pulled 64 at a time per wave from an atomic work counter,
+//   short per-lane stack in LDS (spilling to HBM only when a ray goes deeper than
+//   the LDS slots), interior-node loop with postponed leaves that the wave leaves
+//   only when every lane holds a leaf (wave vote).
+// * shade: one entry per hit; emission, spica's NEE (light sample + BSDF sample
+//   with the power heuristic, mis.cc:21-136), BSDF continuation and Russian
+//   roulette; it appends to the next extend queue, the shadow queue and the MIS
+//   queue with one atomic per wave per queue (ballot + mbcnt prefix).
+// * film: per pixel, the samples of the pass are added in sample order into
+//   Kahan-compensated float sums of L and L^2: deterministic and reproducible.
+#include "spk_internal.h"
+#include "spk_sampler.h"
+
+namespace spk {
+
+namespace {
+
+constexpr float kEPS = 1.0e-12f;      // core/common.h:55
+constexpr float kINFTY = 1.0e32f;     // core/common.h:54
+constexpr float kPI = 3.14159265358979323846f;
+constexpr float kINV_PI = 0.31830988618379067154f;
+constexpr float kRobust = 1.0000005f; // widens the slab interval by ~4 ulp
+constexpr int32_t kSent = 0x7fffffff; // "no node"
+
+// ---- small vector helpers ---------------------------------------------------------
+__device__ __forceinline__ float3 mk(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 add(float3 a, float3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub(float3 a, float3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul(float3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 cmul(float3 a, float3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float3 neg(float3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 normalize(float3 a) {
+    float n = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    return mk(a.x / n, a.y / n, a.z / n);
+}
+__device__ __forceinline__ bool black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ float gray(float3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ float comp(float3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// float.h:23-45 in single precision (zero goes down for Up, up for Down).
+__device__ __forceinline__ float next_up(float f) {
+    if (__builtin_isinf(f) && f > 0.0f) return f;
+    if (f == -0.0f) f = 0.0f;
+    uint32_t u = __float_as_uint(f);
+    u = f > 0.0f ? u + 1u : u - 1u;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ float next_down(float f) {
+    if (__builtin_isinf(f) && f > 0.0f) return f;
+    if (f == -0.0f) f = 0.0f;
+    uint32_t u = __float_as_uint(f);
+    u = f > 0.0f ? u - 1u : u + 1u;
+    return __uint_as_float(u);
+}
+// interaction.cc:16-30
+__device__ __forceinline__ float3 offset_origin(float3 p, float3 n, float3 w) {
+    float3 off = mul(n, 1.0e-3f);
+    if (dot(w, n) < 0.0f) off = neg(off);
+    float3 po = add(p, off);
+    return mk(off.x > 0.0f ? next_up(po.x) : next_down(po.x), off.y > 0.0f ? next_up(po.y) : next_down(po.y),
+              off.z > 0.0f ? next_up(po.z) : next_down(po.z));
+}
+// vect_math.h:121-128
+__device__ __forceinline__ void coordinate_system(float3 w, float3& u, float3& v) {
+    if (fabsf(w.x) > fabsf(w.y)) u = normalize(mk(-w.z, 0.0f, w.x));
+    else u = normalize(mk(0.0f, w.z, -w.y));
+    v = normalize(cross(w, u));
+}
+
+// ---- wave-level queue append ----------------------------------------------------------
+// All 64 lanes of the wave must call this together (the loops below are uniform).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
+    unsigned long long m = __ballot(pred ? 1 : 0);
+    uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    uint32_t total = (uint32_t)__popcll(m);
+    uint32_t base = 0;
+    if (__lane_id() == 0 && total != 0) base = atomicAdd(counter, total);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    return base + prefix;
+}
+
+// ---- ray/triangle (triangle.cc:98-178 in float) -----------------------------------------
+__device__ __forceinline__ bool tri_hit(const float4* __restrict__ tris, uint32_t k, float3 o, float3 d, float tmax,
+                                        float& t, float& u, float& v) {
+    float4 T0 = tris[3 * k], T1 = tris[3 * k + 1], T2 = tris[3 * k + 2];
+    float3 e1 = xyz(T1), e2 = xyz(T2);
+    float3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (det > -kEPS && det < kEPS) return false;
+    float inv = 1.0f / det;
+    float3 tv = sub(o, xyz(T0));
+    float uu = dot(tv, pv) * inv;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    float3 qv = cross(tv, e1);
+    float vv = dot(d, qv) * inv;
+    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    float tt = dot(e2, qv) * inv;
+    if (tt <= kEPS || tt > tmax) return false;
+    t = tt;
+    u = uu;
+    v = vv;
+    return true;
+}
+
+__device__ __forceinline__ float3 safe_inv(float3 d) {
+    // ray.cc:43-47: zero components become INFTY; tiny ones are clamped to it
+    auto one = [](float x) {
+        if (x == 0.0f) return kINFTY;
+        float r = 1.0f / x;
+        return fabsf(r) > kINFTY ? copysignf(kINFTY, r) : r;
+    };
+    return mk(one(d.x), one(d.y), one(d.z));
+}
+
+struct Counters {
+    uint32_t nodes, tris;
+};
+
+// Closest hit (ANY = false) or first hit (ANY = true).  Returns the leaf-order
+// triangle index or -1.  `col` is this lane's LDS stack column (slot s at
+// col[s * BLOCK]); `spill` holds entries below the S most recent ones.
+template <int BLOCK, int S, bool ANY, bool COUNT>
+__device__ __forceinline__ int32_t traverse(const SceneDev& sc, float3 o, float3 d, float tmax, float& t_out, float& u_out,
+                                            float& v_out, uint32_t* col, uint32_t* spill, bool active, Counters& cnt) {
+    const float4* __restrict__ nodes = sc.nodes;
+    const float4* __restrict__ tris = sc.tris;
+    float3 inv = safe_inv(d);
+    int32_t node = (active && sc.has_bvh) ? 0 : kSent;
+    int32_t leaf = 0;  // pending leaf link (negative) or 0
+    int sp = 0, spilled = 0;
+    int32_t hit = -1;
+    float tbest = tmax, hu = 0.0f, hv = 0.0f;
+
+    auto push = [&](int32_t x) {
+        if (sp - spilled == S) {
+            spill[spilled] = col[(spilled & (S - 1)) * BLOCK];
+            spilled++;
+        }
+        col[(sp & (S - 1)) * BLOCK] = (uint32_t)x;
+        sp++;
+    };
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kSent;
+        sp--;
+        if (sp < spilled) {
+            spilled = sp;
+            return (int32_t)spill[sp];
+        }
+        return (int32_t)col[(sp & (S - 1)) * BLOCK];
+    };
+
+    for (;;) {
+        // Interior nodes until every lane of the wave holds a leaf (or is done).
+        while (node >= 0 && node != kSent) {
+            const float4* nd = nodes + 4 * node;
+            float4 a = nd[0], b = nd[1], c = nd[2];
+            float4 lk = nd[3];
+            if (COUNT) cnt.nodes++;
+            float l0x = (a.x - o.x) * inv.x, h0x = (a.y - o.x) * inv.x;
+            float l0y = (a.z - o.y) * inv.y, h0y = (a.w - o.y) * inv.y;
+            float l0z = (c.x - o.z) * inv.z, h0z = (c.y - o.z) * inv.z;
+            float l1x = (b.x - o.x) * inv.x, h1x = (b.y - o.x) * inv.x;
+            float l1y = (b.z - o.y) * inv.y, h1y = (b.w - o.y) * inv.y;
+            float l1z = (c.z - o.z) * inv.z, h1z = (c.w - o.z) * inv.z;
+            float tn0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), 0.0f));
+            float tf0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tbest));
+            float tn1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), 0.0f));
+            float tf1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tbest));
+            bool h0 = tn0 <= tf0 * kRobust;
+            bool h1 = tn1 <= tf1 * kRobust;
+            int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y);
+            if (h0 && h1) {
+                bool first0 = tn0 <= tn1;
+                node = first0 ? c0 : c1;
+                push(first0 ? c1 : c0);
+            } else if (h0) {
+                node = c0;
+            } else if (h1) {
+                node = c1;
+            } else {
+                node = pop();
+            }
+            if (node < 0 && leaf == 0) {
+                leaf = node;
+                node = pop();
+            }
+            if (__all(leaf != 0)) break;
+        }
+        // Leaves.
+        while (leaf < 0) {
+            uint32_t x = ~(uint32_t)leaf;
+            uint32_t first = x >> kLeafCountBits, n = (x & (kMaxLeafTris - 1)) + 1;
+            for (uint32_t k = first; k < first + n; k++) {
+                float t, u, v;
+                if (COUNT) cnt.tris++;
+                if (tri_hit(tris, k, o, d, tbest, t, u, v)) {
+                    hit = (int32_t)k;
+                    if (ANY) break;
+                    tbest = t;
+                    hu = u;
+                    hv = v;
+                }
+            }
+            if (ANY && hit >= 0) {
+                node = kSent;
+                leaf = 0;
+                sp = 0;
+                spilled = 0;
+                break;
+            }
+            leaf = 0;
+            if (node < 0) {
+                leaf = node;
+                node = pop();
+            }
+        }
+        if (node == kSent && leaf == 0) break;
+    }
+    t_out = tbest;
+    u_out = hu;
+    v_out = hv;
+    return hit;
+}
+
+// Persistent-thread work fetch: 64 items per wave per atomic.
+__device__ __forceinline__ uint32_t fetch_work(uint32_t* counter) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+}
+
+// ---- standalone trace (spk_trace) --------------------------------------------------------
+template <int BLOCK, int S, bool ANY>
+__global__ __launch_bounds__(BLOCK) void k_trace(SceneDev sc, TraceBufs b, uint32_t* work, uint32_t* spill_all,
+                                                 int spill_cap) {
+    __shared__ uint32_t lds[S * BLOCK];
+    uint32_t* col = lds + threadIdx.x;
+    uint32_t* spill = spill_all + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * spill_cap;
+    const uint64_t n = (uint64_t)b.n;
+    for (;;) {
+        uint64_t base = fetch_work(work);
+        if (base >= n) break;
+        uint64_t i = base + __lane_id();
+        bool active = i < n;
+        float3 o = mk(0, 0, 1), d = mk(0, 0, 1);
+        float tmax = kINFTY;
+        if (active) {
+            o = mk(b.org[3 * i], b.org[3 * i + 1], b.org[3 * i + 2]);
+            d = normalize(mk(b.dir[3 * i], b.dir[3 * i + 1], b.dir[3 * i + 2]));
+            if (b.tmax) tmax = b.tmax[i];
+        }
+        float t, u, v;
+        Counters cnt = {0, 0};
+        int32_t k = traverse<BLOCK, S, ANY, false>(sc, o, d, tmax, t, u, v, col, spill, active, cnt);
+        if (active) {
+            if (ANY) {
+                b.prim_out[i] = k >= 0 ? 1 : 0;
+            } else {
+                b.prim_out[i] = k >= 0 ? sc.tri_ids[k].x : -1;
+                if (b.t_out) b.t_out[i] = k >= 0 ? t : -1.0f;
+            }
+        }
+    }
+}
+
+// ---- pass bookkeeping -----------------------------------------------------------------------
+__global__ void k_set_pass(PassState* ps, uint32_t index) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ps->pass_index = index;
+}
+
+__global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps) {
+    // One block: thread 0 computes the pass, all threads clear the counters.
+    __shared__ uint32_t n_sh;
+    if (threadIdx.x == 0) {
+        uint32_t j = ps->pass_index;
+        uint32_t total = cfg->end_sample - cfg->first_sample;
+        uint32_t sample_base, pixel_base, n_pix, k;
+        bool done;
+        if (cfg->chunks > 1) {
+            uint32_t s = j / cfg->chunks, c = j % cfg->chunks;
+            done = s >= total;
+            sample_base = cfg->first_sample + s;
+            pixel_base = c * cfg->queue;
+            n_pix = min(cfg->queue, cfg->npix - pixel_base);
+            k = 1;
+        } else {
+            uint32_t s0 = j * cfg->kmax;
+            done = s0 >= total;
+            sample_base = cfg->first_sample + s0;
+            pixel_base = 0;
+            n_pix = cfg->npix;
+            k = done ? 0 : min(cfg->kmax, total - s0);
+        }
+        ps->done = done ? 1u : 0u;
+        ps->sample_base = sample_base;
+        ps->pixel_base = pixel_base;
+        ps->n_pix = done ? 0u : n_pix;
+        ps->k_samples = done ? 0u : k;
+        ps->n = done ? 0u : n_pix * k;
+        ps->pass_index = j + 1;
+        n_sh = ps->n;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMaxDepthCap + 2; i += blockDim.x) {
+        ps->countA[i] = 0;
+        ps->countS[i] = 0;
+        ps->countM[i] = 0;
+        ps->workA[i] = 0;
+        ps->workS[i] = 0;
+        ps->workM[i] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ps->countA[0] = n_sh;
+}
+
+// Pixel-order index -> raster (x, y): bands of 8 rows, 8-column blocks inside a
+// band, so one wave of 64 consecutive paths covers an 8x8 tile.
+__device__ __forceinline__ void swizzle(uint32_t q, int W, int H, int& x, int& y) {
+    uint32_t band = q / (8u * (uint32_t)W);
+    uint32_t rows = min(8u, (uint32_t)H - 8u * band);
+    uint32_t r = q - band * 8u * (uint32_t)W;
+    uint32_t blk = r / (8u * rows);
+    uint32_t cols = min(8u, (uint32_t)W - 8u * blk);
+    uint32_t within = r - blk * 8u * rows;
+    x = (int)(8u * blk + within % cols);
+    y = (int)(8u * band + within / cols);
+}
+
+// ---- camera (cameras/perspective.cc:53-74, integrator.cc:84-86) ---------------------------
+__global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cfg, RenderBufs rb) {
+    const PassState* ps = rb.ps;
+    const uint32_t n = ps->n;
+    const uint32_t n_pix = ps->n_pix, pixel_base = ps->pixel_base, sample_base = ps->sample_base;
+    const int W = cfg->W, H = cfg->H;
+    const CameraDev cam = cfg->cam;
+    const int kind = cfg->sampler;
+    const uint32_t seed = cfg->seed;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t s_local = i / n_pix, j = i - s_local * n_pix;
+        int x, y;
+        swizzle(pixel_base + j, W, H, x, y);
+        uint32_t pixel = (uint32_t)(y * W + x), sample = sample_base + s_local;
+        float2 uf = sample_2d(kind, seed, pixel, sample, 0);
+        float sx = 2.0f * ((float)x + uf.x) / (float)W - 1.0f;
+        float sy = 1.0f - 2.0f * ((float)y + uf.y) / (float)H;
+        float3 dir = normalize(mk(sx * cam.sx_scale, sy * cam.sy_scale, 1.0f));
+        float3 org = mk(0, 0, 0);
+        if (cam.lens_r > 0.0f) {
+            float2 ul = sample_2d(kind, seed, pixel, sample, 2);
+            // sampling.cc:151-164 concentric disk
+            float ox = 2.0f * ul.x - 1.0f, oy = 2.0f * ul.y - 1.0f, dx = 0.0f, dy = 0.0f;
+            if (ox != 0.0f || oy != 0.0f) {
+                float r, th;
+                if (fabsf(ox) > fabsf(oy)) {
+                    r = ox;
+                    th = kPI * (oy / ox) / 4.0f;
+                } else {
+                    r = oy;
+                    th = (kPI / 2.0f) - kPI * (ox / oy) / 4.0f;
+                }
+                dx = r * cosf(th);
+                dy = r * sinf(th);
+            }
+            float ft = cam.focal / dir.z;
+            float3 pf = mul(dir, ft);
+            org = mk(dx * cam.lens_r, dy * cam.lens_r, 0.0f);
+            dir = normalize(sub(pf, org));
+        }
+        const float* m = cam.m;
+        float3 ow = mk(m[0] * org.x + m[1] * org.y + m[2] * org.z + m[3], m[4] * org.x + m[5] * org.y + m[6] * org.z + m[7],
+                       m[8] * org.x + m[9] * org.y + m[10] * org.z + m[11]);
+        float3 dw = normalize(mk(m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[4] * dir.x + m[5] * dir.y + m[6] * dir.z,
+                                 m[8] * dir.x + m[9] * dir.y + m[10] * dir.z));
+        rb.qa_o[0][i] = make_float4(ow.x, ow.y, ow.z, __uint_as_float(i));
+        rb.qa_d[0][i] = make_float4(dw.x, dw.y, dw.z, __uint_as_float(s_local << 7));
+        rb.qa_b[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(pixel));
+        rb.path_L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// ---- extend: closest hit over the extend queue -------------------------------------------------
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, int depth) {
+    __shared__ uint32_t lds[S * BLOCK];
+    PassState* ps = rb.ps;
+    const uint32_t count = ps->countA[depth];
+    if (count == 0) return;
+    uint32_t* col = lds + threadIdx.x;
+    uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
+    const float4* __restrict__ qo = rb.qa_o[depth & 1];
+    const float4* __restrict__ qd = rb.qa_d[depth & 1];
+    for (;;) {
+        uint32_t base = fetch_work(&ps->workA[depth]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        bool active = i < count;
+        float3 o = mk(0, 0, 0), d = mk(0, 0, 1);
+        if (active) {
+            o = xyz(qo[i]);
+            d = xyz(qd[i]);
+        }
+        float t, u, v;
+        Counters cnt = {0, 0};
+        int32_t k = traverse<BLOCK, S, false, false>(sc, o, d, kINFTY, t, u, v, col, spill, active, cnt);
+        if (active) rb.hit[i] = make_float4(t, u, v, __int_as_float(k));
+    }
+}
+
+// ---- BSDF (core/bsdf.cc, core/bxdf.cc, core/microfacet.cc, core/fresnel.cc) -------------------
+enum { BX_REFLECTION = 1, BX_TRANSMISSION = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 16, BX_ALL = 31 };
+
+struct Bsdf {
+    int lobe, type;
+    float3 R, T, eta, k;
+    float etaB, alpha;
+    float3 t, b, n;  // frame
+};
+
+__device__ __forceinline__ int lobe_type(int lobe) {
+    switch (lobe) {
+    case LOBE_LAMBERT: return BX_REFLECTION | BX_DIFFUSE;
+    case LOBE_SPECREFL: return BX_REFLECTION | BX_SPECULAR;
+    case LOBE_FRESNELSPEC: return BX_REFLECTION | BX_TRANSMISSION | BX_SPECULAR;
+    case LOBE_MICROREFL: return BX_REFLECTION | BX_GLOSSY;
+    default: return 0;
+    }
+}
+
+__device__ __forceinline__ float3 fr_conductor(float cosI, float3 eta, float3 kk) {
+    cosI = clampf(cosI, -1.0f, 1.0f);
+    float r[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float e = comp(eta, c), ek = comp(kk, c);
+        float cos2 = cosI * cosI, sin2 = 1.0f - cos2;
+        float e2 = e * e, ek2 = ek * ek;
+        float t0 = e2 - ek2 - sin2;
+        float a2b2 = sqrtf(t0 * t0 + 4.0f * e2 * ek2);
+        float t1 = a2b2 + cos2;
+        float a = sqrtf(0.5f * (a2b2 + t0));
+        float t2 = 2.0f * cosI * a;
+        float Rs = (t1 - t2) / (t1 + t2);
+        float t3 = cos2 * a2b2 + sin2 * sin2;
+        float t4 = t2 * sin2;
+        float Rp = Rs * (t3 - t4) / (t3 + t4 + kEPS);
+        r[c] = 0.5f * (Rp + Rs);
+    }
+    return mk(r[0], r[1], r[2]);
+}
+
+__device__ __forceinline__ float fr_dielectric(float cosI, float etaI, float etaT) {
+    cosI = clampf(cosI, -1.0f, 1.0f);
+    if (!(cosI > 0.0f)) {
+        float tmp = etaI;
+        etaI = etaT;
+        etaT = tmp;
+        cosI = fabsf(cosI);
+    }
+    float sinI = sqrtf(fmaxf(0.0f, 1.0f - cosI * cosI));
+    float sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0f) return 1.0f;
+    float cosT = sqrtf(fmaxf(0.0f, 1.0f - sinT * sinT));
+    float rpar = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+    float rper = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+    return 0.5f * (rpar * rpar + rper * rper);
+}
+
+__device__ __forceinline__ float sin2t(float3 w) { return fmaxf(0.0f, 1.0f - w.z * w.z); }
+__device__ __forceinline__ float cos_phi(float3 w) {
+    float s = sqrtf(sin2t(w));
+    return s == 0.0f ? 1.0f : clampf(w.x / s, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float sin_phi(float3 w) {
+    float s = sqrtf(sin2t(w));
+    return s == 0.0f ? 0.0f : clampf(w.y / s, -1.0f, 1.0f);
+}
+__device__ __forceinline__ float ggx_D(float alpha, float3 wh) {
+    float c2 = wh.z * wh.z;
+    float t2 = sin2t(wh) / (c2 + kEPS);
+    if (__builtin_isinf(t2)) return 0.0f;
+    float cp = cos_phi(wh), sp = sin_phi(wh);
+    float e = (cp * cp / (alpha * alpha) + sp * sp / (alpha * alpha)) * t2;
+    return 1.0f / (kPI * alpha * alpha * c2 * c2 * (1.0f + e) * (1.0f + e));
+}
+__device__ __forceinline__ float ggx_lambda(float alpha, float3 w) {
+    float at = fabsf(sqrtf(sin2t(w)) / (w.z + kEPS));
+    if (__builtin_isinf(at)) return 0.0f;
+    float cp = cos_phi(w), sp = sin_phi(w);
+    float al = sqrtf(cp * cp * alpha * alpha + sp * sp * alpha * alpha);
+    float a = al * at;
+    return (-1.0f + sqrtf(1.0f + a * a)) / 2.0f;
+}
+__device__ __forceinline__ float ggx_G1(float alpha, float3 w, float3 wh) {
+    if (w.z * dot(w, wh) <= 0.0f) return 0.0f;
+    return 1.0f / (1.0f + ggx_lambda(alpha, w));
+}
+__device__ __forceinline__ float ggx_G(float alpha, float3 wo, float3 wi, float3 wh) {
+    if (wi.z * dot(wi, wh) <= 0.0f) return 0.0f;
+    if (wo.z * dot(wo, wh) <= 0.0f) return 0.0f;
+    return 1.0f / (1.0f + ggx_lambda(alpha, wo) + ggx_lambda(alpha, wi));
+}
+__device__ __forceinline__ float ggx_pdf(float alpha, float3 wo, float3 wh) {
+    return ggx_D(alpha, wh) * ggx_G1(alpha, wo, wh) * fabsf(dot(wo, wh)) / fabsf(wo.z);
+}
+// microfacet.cc:15-56: visible-slope sampling, rational fit for the y slope.
+__device__ __forceinline__ void ggx_sample11(float cosT, float u0, float u1, float& sx, float& sy) {
+    if (cosT > 0.9999f) {
+        float r = sqrtf(u0 / (1.0f - u0));
+        float phi = 2.0f * kPI * u1;
+        sx = r * cosf(phi);
+        sy = r * sinf(phi);
+        return;
+    }
+    float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+    float tanT = sinT / cosT;
+    float a = 1.0f / tanT;
+    float G1 = 2.0f / (1.0f + sqrtf(1.0f + 1.0f / (a * a)));
+    float A = 2.0f * u0 / G1 - 1.0f;
+    float tmp = 1.0f / (A * A - 1.0f);
+    if (tmp > 1.0e10f) tmp = 1.0e10f;
+    float B = tanT;
+    float D = sqrtf(fmaxf(0.0f, B * B * tmp * tmp - (A * A - B * B) * tmp));
+    float x1 = B * tmp - D, x2 = B * tmp + D;
+    sx = (A < 0.0f || x2 > 1.0f / tanT) ? x1 : x2;
+    float S, U;
+    if (u1 > 0.5f) {
+        S = 1.0f;
+        U = 2.0f * (u1 - 0.5f);
+    } else {
+        S = -1.0f;
+        U = 2.0f * (0.5f - u1);
+    }
+    float z = (U * (U * (U * 0.27385f - 0.73369f) + 0.46341f)) / (U * (U * (U * 0.093073f + 0.309420f) - 1.0f) + 0.597999f);
+    sy = S * z * sqrtf(1.0f + sx * sx);
+}
+__device__ __forceinline__ float3 ggx_sample_wh(float alpha, float3 wo, float u0, float u1) {
+    bool flip = wo.z < 0.0f;
+    float3 wi = flip ? neg(wo) : wo;
+    float3 ws = normalize(mk(alpha * wi.x, alpha * wi.y, wi.z));
+    float sx, sy;
+    ggx_sample11(ws.z, u0, u1, sx, sy);
+    float cp = cos_phi(ws), sp = sin_phi(ws);
+    float tmp = cp * sx - sp * sy;
+    sy = sp * sx + cp * sy;
+    sx = tmp * alpha;
+    sy = sy * alpha;
+    float3 wh = normalize(mk(-sx, -sy, 1.0f));
+    return flip ? neg(wh) : wh;
+}
+__device__ __forceinline__ float3 micro_f(const Bsdf& b, float3 wo, float3 wi) {
+    float cO = fabsf(wo.z), cI = fabsf(wi.z);
+    float3 wh = add(wi, wo);
+    if (cI == 0.0f || cO == 0.0f) return mk(0, 0, 0);
+    if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk(0, 0, 0);
+    wh = normalize(wh);
+    float3 F = fr_conductor(fabsf(dot(wi, wh)), b.eta, b.k);
+    float s = ggx_D(b.alpha, wh) * ggx_G(b.alpha, wo, wi, wh) / (4.0f * cI * cO);
+    return mul(cmul(b.R, F), s);
+}
+__device__ __forceinline__ bool refract_v(float3 wi, float3 n, float eta, float3& wt) {
+    float cosI = dot(n, wi);
+    float sin2I = fmaxf(0.0f, 1.0f - cosI * cosI);
+    float sin2T = eta * eta * sin2I;
+    if (sin2T >= 1.0f) return false;
+    float cosT = sqrtf(1.0f - sin2T);
+    wt = add(mul(neg(wi), eta), mul(n, eta * cosI - cosT));
+    return true;
+}
+
+__device__ float3 lobe_f(const Bsdf& b, float3 wo, float3 wi) {
+    const float deltaEps = 1.0e-4f;
+    switch (b.lobe) {
+    case LOBE_LAMBERT:
+        return mul(b.R, kINV_PI);
+    case LOBE_SPECREFL:
+        if (dot(mk(-wo.x, -wo.y, wo.z), wi) > 1.0f - deltaEps) return mul(b.R, 1.0f / fabsf(wi.z));
+        return mk(0, 0, 0);
+    case LOBE_FRESNELSPEC: {
+        float F = fr_dielectric(wo.z, 1.0f, b.etaB);
+        if (wo.z * wi.z > 0.0f) {
+            if (dot(mk(-wo.x, -wo.y, wo.z), wi) > 1.0f - deltaEps) return mul(b.R, F / fabsf(wi.z));
+        } else {
+            bool entering = wo.z > 0.0f;
+            float etaI = entering ? 1.0f : b.etaB, etaT = entering ? b.etaB : 1.0f;
+            float3 wt;
+            if (!refract_v(wo, wo.z < 0.0f ? mk(0, 0, -1) : mk(0, 0, 1), etaI / etaT, wt)) return mk(0, 0, 0);
+            if (dot(wi, wt) > 1.0f - deltaEps) return mul(b.T, (1.0f - F) / fabsf(wi.z));
+        }
+        return mk(0, 0, 0);
+    }
+    case LOBE_MICROREFL:
+        return micro_f(b, wo, wi);
+    default:
+        return mk(0, 0, 0);
+    }
+}
+__device__ float lobe_pdf(const Bsdf& b, float3 wo, float3 wi) {
+    const float deltaEps = 1.0e-4f;
+    switch (b.lobe) {
+    case LOBE_LAMBERT:
+        return wo.z * wi.z > 0.0f ? fabsf(wi.z) * kINV_PI : 0.0f;
+    case LOBE_SPECREFL:
+        return dot(mk(-wo.x, -wo.y, wo.z), wi) > 1.0f - deltaEps ? 1.0f : 0.0f;
+    case LOBE_FRESNELSPEC: {
+        float F = fr_dielectric(wo.z, 1.0f, b.etaB);
+        if (wo.z * wi.z > 0.0f) {
+            if (dot(mk(-wo.x, -wo.y, wo.z), wi) > 1.0f - deltaEps) return F;
+        } else {
+            bool entering = wo.z > 0.0f;
+            float etaI = entering ? 1.0f : b.etaB, etaT = entering ? b.etaB : 1.0f;
+            float3 wt;
+            if (!refract_v(wo, wo.z < 0.0f ? mk(0, 0, -1) : mk(0, 0, 1), etaI / etaT, wt)) return 0.0f;
+            if (dot(wi, wt) > 1.0f - deltaEps) return 1.0f - F;
+        }
+        return 0.0f;
+    }
+    case LOBE_MICROREFL: {
+        if (!(wo.z * wi.z > 0.0f)) return 0.0f;
+        float3 wh = normalize(add(wo, wi));
+        return ggx_pdf(b.alpha, wo, wh) / (4.0f * dot(wo, wh));
+    }
+    default:
+        return 0.0f;
+    }
+}
+// Returns f; pdf stays 0 on failure; stype receives the sampled component type.
+__device__ float3 lobe_sample(const Bsdf& b, float3 wo, float3& wi, float u0, float u1, float& pdf, int& stype) {
+    switch (b.lobe) {
+    case LOBE_LAMBERT: {
+        float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f, dx = 0.0f, dy = 0.0f;
+        if (ox != 0.0f || oy != 0.0f) {
+            float r, th;
+            if (fabsf(ox) > fabsf(oy)) {
+                r = ox;
+                th = kPI * (oy / ox) / 4.0f;
+            } else {
+                r = oy;
+                th = (kPI / 2.0f) - kPI * (ox / oy) / 4.0f;
+            }
+            dx = r * cosf(th);
+            dy = r * sinf(th);
+        }
+        wi = mk(dx, dy, sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy)));
+        if (wo.z < 0.0f) wi.z *= -1.0f;
+        pdf = lobe_pdf(b, wo, wi);
+        return lobe_f(b, wo, wi);
+    }
+    case LOBE_SPECREFL: {
+        wi = mk(-wo.x, -wo.y, wo.z);
+        pdf = 1.0f;
+        float3 F = fr_conductor(fabsf(wi.z), b.eta, b.k);
+        return mul(cmul(F, b.R), 1.0f / fabsf(wi.z));
+    }
+    case LOBE_FRESNELSPEC: {
+        float F = fr_dielectric(wo.z, 1.0f, b.etaB);
+        if (u0 < F) {
+            wi = mk(-wo.x, -wo.y, wo.z);
+            stype = BX_SPECULAR | BX_REFLECTION;
+            pdf = F;
+            return mul(b.R, F / fabsf(wi.z));
+        }
+        bool entering = wo.z > 0.0f;
+        float etaI = entering ? 1.0f : b.etaB, etaT = entering ? b.etaB : 1.0f;
+        if (!refract_v(wo, wo.z < 0.0f ? mk(0, 0, -1) : mk(0, 0, 1), etaI / etaT, wi)) return mk(0, 0, 0);
+        float3 ft = mul(b.T, (1.0f - F) * ((etaI * etaI) / (etaT * etaT)));
+        stype = BX_SPECULAR | BX_TRANSMISSION;
+        pdf = 1.0f - F;
+        return mul(ft, 1.0f / fabsf(wi.z));
+    }
+    case LOBE_MICROREFL: {
+        if (wo.z == 0.0f) return mk(0, 0, 0);
+        float3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+        wi = sub(mul(wh, 2.0f * dot(wh, wo)), wo);
+        if (!(wo.z * wi.z > 0.0f)) return mk(0, 0, 0);
+        pdf = ggx_pdf(b.alpha, wo, wh) / (4.0f * dot(wo, wh));
+        return micro_f(b, wo, wi);
+    }
+    default:
+        return mk(0, 0, 0);
+    }
+}
+__device__ __forceinline__ float3 to_local(const Bsdf& b, float3 v) { return mk(dot(b.t, v), dot(b.b, v), dot(b.n, v)); }
+__device__ __forceinline__ float3 to_world(const Bsdf& b, float3 v) {
+    return add(add(mul(b.t, v.x), mul(b.b, v.y)), mul(b.n, v.z));
+}
+__device__ __forceinline__ bool bsdf_has_nonspecular(const Bsdf& b) {
+    return b.lobe == LOBE_LAMBERT || b.lobe == LOBE_MICROREFL;
+}
+// bsdf.cc:49-63
+__device__ float3 bsdf_f(const Bsdf& b, float3 woW, float3 wiW, int mask) {
+    if (b.lobe == LOBE_NONE) return mk(0, 0, 0);
+    bool reflect = dot(wiW, b.n) * dot(woW, b.n) > 0.0f;
+    if ((b.type & mask) != 0 && ((reflect && (b.type & BX_REFLECTION)) || (!reflect && (b.type & BX_TRANSMISSION))))
+        return lobe_f(b, to_local(b, woW), to_local(b, wiW));
+    return mk(0, 0, 0);
+}
+// bsdf.cc:124-136
+__device__ float bsdf_pdf(const Bsdf& b, float3 woW, float3 wiW, int mask) {
+    if (b.lobe == LOBE_NONE || (b.type & mask) == 0) return 0.0f;
+    return lobe_pdf(b, to_local(b, woW), to_local(b, wiW));
+}
+// bsdf.cc:65-122 (one BxDF)
+__device__ float3 bsdf_sample(const Bsdf& b, float3 woW, float3& wiW, float u0, float u1, float& pdf, int mask, int& stype) {
+    pdf = 0.0f;
+    stype = 0;
+    if (b.lobe == LOBE_NONE || (b.type & mask) != b.type) return mk(0, 0, 0);
+    float3 wo = normalize(to_local(b, woW));
+    if (wo.z == 0.0f) return mk(0, 0, 0);
+    stype = b.type;
+    float3 wi = mk(0, 0, 1);
+    float3 f = lobe_sample(b, wo, wi, u0, u1, pdf, stype);
+    if (pdf == 0.0f) {
+        stype = 0;
+        return mk(0, 0, 0);
+    }
+    wiW = normalize(to_world(b, wi));
+    return f;
+}
+
+// ---- environment map (lights/envmap.cc, core/mipmap.cc, core/sampling.cc) ----------------------
+__device__ __forceinline__ float3 env_texel(const EnvDev& e, int s, int t) {
+    s = (s % e.w + e.w) % e.w;
+    t = (t % e.h + e.h) % e.h;
+    return xyz(e.tex[t * e.w + s]);
+}
+// mipmap.cc:71-79 at level 0 (truncation toward zero, as spica does)
+__device__ float3 env_lookup(const EnvDev& e, float u, float v) {
+    float s = u * (float)e.w - 0.5f, t = v * (float)e.h - 0.5f;
+    int si = (int)s, ti = (int)t;
+    float ds = s - (float)si, dt = t - (float)ti;
+    float3 r = mul(env_texel(e, si, ti), (1.0f - ds) * (1.0f - dt));
+    r = add(r, mul(env_texel(e, si + 1, ti), ds * (1.0f - dt)));
+    r = add(r, mul(env_texel(e, si, ti + 1), (1.0f - ds) * dt));
+    r = add(r, mul(env_texel(e, si + 1, ti + 1), ds * dt));
+    return r;
+}
+__device__ __forceinline__ float3 m3(const float* m, float3 v) {
+    return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z, m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+__device__ __forceinline__ float sph_phi(float3 d) {
+    float p = atan2f(d.y, d.x);
+    return p < 0.0f ? p + 2.0f * kPI : p;
+}
+// envmap.cc:130-134
+__device__ float3 env_Le(const EnvDev& e, float3 dir) {
+    float3 d = normalize(m3(e.w2l, dir));
+    return env_lookup(e, sph_phi(d) * (0.5f * kINV_PI), acosf(clampf(d.z, -1.0f, 1.0f)) * kINV_PI);
+}
+// envmap.cc:81-88
+__device__ float env_pdf(const EnvDev& e, float3 dir) {
+    float3 d = m3(e.w2l, dir);
+    float theta = acosf(clampf(d.z, -1.0f, 1.0f));
+    float phi = sph_phi(d);
+    float st = sinf(theta);
+    if (st == 0.0f) return 0.0f;
+    int iu = min(max((int)(phi * (0.5f * kINV_PI) * (float)e.w), 0), e.w - 1);
+    int iv = min(max((int)(theta * kINV_PI * (float)e.h), 0), e.h - 1);
+    return e.pdf2d[iv * e.w + iu] / (2.0f * kPI * kPI * st);
+}
+// sampling.cc:94-97: upper_bound - 1 over cdf[0..size)
+__device__ __forceinline__ int find_interval(const float* cdf, int size, float v) {
+    int lo = 0, hi = size;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (v < cdf[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    return min(lo - 1, size - 1);
+}
+// envmap.cc:60-79
+__device__ float3 env_sample(const EnvDev& e, float u0, float u1, float3& dir, float& pdf, float& uu, float& vv) {
+    int vo = find_interval(e.marg_cdf, e.h + 1, u1);
+    float c0 = e.marg_cdf[vo], c1 = e.marg_cdf[vo + 1];
+    float du = u1 - c0;
+    if (c1 - c0 > 0.0f) du /= (c1 - c0);
+    float p1 = e.marg_pdf[vo];
+    vv = ((float)vo + du) / (float)e.h;
+    const float* rc = e.cond_cdf + (size_t)vo * (e.w + 1);
+    int uo = find_interval(rc, e.w + 1, u0);
+    float d0 = rc[uo], d1 = rc[uo + 1];
+    float dv = u0 - d0;
+    if (d1 - d0 > 0.0f) dv /= (d1 - d0);
+    float p0 = e.cond_pdf[(size_t)vo * e.w + uo];
+    uu = ((float)uo + dv) / (float)e.w;
+    float mp = p0 * p1;
+    if (mp == 0.0f) {
+        pdf = 0.0f;
+        return mk(0, 0, 0);
+    }
+    float theta = vv * kPI, phi = uu * (2.0f * kPI);
+    float st = sinf(theta), ct = cosf(theta);
+    dir = m3(e.l2w, mk(st * cosf(phi), st * sinf(phi), ct));
+    pdf = st == 0.0f ? 0.0f : mp / (2.0f * kPI * kPI * st);
+    return env_lookup(e, uu, vv);
+}
+
+// Shape::pdf(ref, wi) (core/shape.cc:35-48): re-intersect the light triangle
+// from the offset origin.
+__device__ float area_pdf(const LightDev& L, float3 p, float3 ng, float3 wi) {
+    float3 o = offset_origin(p, ng, wi);
+    float3 e1 = xyz(L.e1), e2 = xyz(L.e2);
+    float3 pv = cross(wi, e2);
+    float det = dot(e1, pv);
+    if (det > -kEPS && det < kEPS) return 0.0f;
+    float inv = 1.0f / det;
+    float3 tv = sub(o, xyz(L.p0));
+    float u = dot(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return 0.0f;
+    float3 qv = cross(tv, e1);
+    float v = dot(wi, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0.0f;
+    float t = dot(e2, qv) * inv;
+    if (t <= kEPS || t > kINFTY) return 0.0f;
+    float3 hp = add(o, mul(wi, t));
+    float3 dd = sub(p, hp);
+    float r = dot(dd, dd) / (fabsf(dot(xyz(L.ng), neg(wi))) * L.p0.w);
+    return __builtin_isinf(r) ? 0.0f : r;
+}
+
+__device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+
+// ---- shade ---------------------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
+                                                 int depth) {
+    PassState* ps = rb.ps;
+    const uint32_t count = ps->countA[depth];
+    if (count == 0) return;
+    const int kind = cfgp->sampler;
+    const uint32_t seed = cfgp->seed;
+    const int max_depth = cfgp->max_depth, rr_depth = cfgp->rr_depth;
+    const uint32_t sample_base = ps->sample_base;
+    const int cur = depth & 1, nxt = cur ^ 1;
+    const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
+    const float nl = (float)sc.num_lights;
+    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < count;
+        bool pushA = false, pushS = false, pushM = false;
+        float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0);
+        uint32_t nA_flags = 0;
+        float3 s_o = mk(0, 0, 0), s_d = mk(0, 0, 1), s_c = mk(0, 0, 0);
+        float s_tmax = 0.0f;
+        float3 m_o = mk(0, 0, 0), m_d = mk(0, 0, 1), m_c = mk(0, 0, 0);
+        int m_light = 0;
+        uint32_t path = 0, pixel = 0;
+        float3 Ladd = mk(0, 0, 0);
+        if (active) {
+            float4 O = rb.qa_o[cur][i], D = rb.qa_d[cur][i], B = rb.qa_b[cur][i], H = rb.hit[i];
+            path = __float_as_uint(O.w);
+            const uint32_t flags = __float_as_uint(D.w);
+            const bool specular = (flags >> 6) & 1u;
+            const uint32_t sample = sample_base + (flags >> 7);
+            pixel = __float_as_uint(B.w);
+            float3 ro = xyz(O), rd = xyz(D), beta = xyz(B);
+            const int32_t k = __float_as_int(H.w);
+            const bool emit = depth == 0 || specular;
+            if (k < 0) {
+                if (emit && sc.env.present) Ladd = cmul(beta, env_Le(sc.env, rd));
+            } else {
+                float4 NG = sc.tri_ng[k];
+                int2 ids = sc.tri_ids[k];
+                float3 ng = xyz(NG);
+                uint32_t mbits = __float_as_uint(NG.w);
+                float3 p = add(ro, mul(rd, H.x));
+                if (emit && ids.y >= 0 && dot(ng, neg(rd)) > 0.0f) Ladd = cmul(beta, xyz(sc.lights[ids.y].L));
+                if (depth < max_depth) {
+                    // BSDF frame (bsdf.cc:14-21 from triangle.cc:127-155)
+                    Bsdf b;
+                    const MatDev M = sc.mats[mbits & 0x7fffffffu];
+                    b.lobe = __float_as_int(M.a.w);
+                    b.type = lobe_type(b.lobe);
+                    b.R = xyz(M.a);
+                    b.T = xyz(M.b);
+                    b.etaB = M.b.w;
+                    b.eta = xyz(M.c);
+                    b.alpha = M.c.w;
+                    b.k = xyz(M.d);
+                    float3 dpdu, dpdv;
+                    coordinate_system(ng, dpdu, dpdv);
+                    b.n = ng;
+                    b.t = dpdu;
+                    b.b = dpdv;
+                    if (mbits >> 31) {
+                        const float4* tn = sc.tri_n + 3 * k;
+                        float3 ns = normalize(add(add(mul(xyz(tn[0]), 1.0f - H.y - H.z), mul(xyz(tn[1]), H.y)),
+                                                  mul(xyz(tn[2]), H.z)));
+                        if (fabsf(dot(ns, ng)) < 1.0f - 1.0e-6f) {
+                            float3 ss = normalize(cross(ns, ng));
+                            float3 ts = normalize(cross(ns, ss));
+                            b.n = normalize(cross(ss, ts));
+                            b.t = ss;
+                            b.b = ts;
+                        }
+                    }
+                    const float3 wo = neg(rd);
+                    // NEE: uniformSampleOneLight + estimateDirectLight (mis.cc:21-136)
+                    if (bsdf_has_nonspecular(b) && sc.num_lights > 0) {
+                        const int mask = BX_ALL & ~BX_SPECULAR;
+                        float upick = sample_1d(kind, seed, pixel, sample, dim_base + 0);
+                        int li = min((int)(upick * nl), sc.num_lights - 1);
+                        float2 ul = sample_2d(kind, seed, pixel, sample, dim_base + 1);
+                        float2 us = sample_2d(kind, seed, pixel, sample, dim_base + 3);
+                        const bool is_env = li >= sc.num_areas;
+                        float3 wi = mk(0, 0, 1), Li = mk(0, 0, 0);
+                        float lightPdf = 0.0f;
+                        float3 lp = mk(0, 0, 0), ln = mk(0, 0, 0);
+                        if (!is_env) {
+                            const LightDev& L = sc.lights[li];
+                            float a0 = ul.x, a1 = ul.y;
+                            if (ul.x + ul.y >= 1.0f) {
+                                a0 = 1.0f - ul.x;
+                                a1 = 1.0f - ul.y;
+                            }
+                            lp = add(add(xyz(L.p0), mul(xyz(L.e1), a0)), mul(xyz(L.e2), a1));
+                            ln = add(add(mul(xyz(L.n0), 1.0f - a0 - a1), mul(xyz(L.n1), a0)), mul(xyz(L.n2), a1));
+                            wi = normalize(sub(lp, p));
+                            lightPdf = area_pdf(L, p, ng, wi);
+                            if (dot(ln, neg(wi)) > 0.0f) Li = xyz(L.L);
+                        } else {
+                            float uu, vv;
+                            Li = env_sample(sc.env, ul.x, ul.y, wi, lightPdf, uu, vv);
+                            lp = add(p, mul(wi, 2.0f * sc.env.radius));
+                        }
+                        if (lightPdf > 0.0f && !black(Li)) {
+                            float3 f = mul(bsdf_f(b, wo, wi, mask), fabsf(dot(wi, b.n)));
+                            float bsdfPdf = bsdf_pdf(b, wo, wi, mask);
+                            if (!black(f)) {
+                                // spawnRayTo(Interaction) (interaction.cc:88-93)
+                                float3 so = offset_origin(p, ng, sub(lp, p));
+                                float3 tg = offset_origin(lp, ln, sub(so, lp));
+                                float3 dv = sub(tg, so);
+                                float dist = sqrtf(dot(dv, dv));
+                                float w = power_heuristic(lightPdf, bsdfPdf);
+                                pushS = true;
+                                s_o = so;
+                                s_d = mk(dv.x / dist, dv.y / dist, dv.z / dist);
+                                s_tmax = dist;
+                                s_c = mul(cmul(beta, cmul(f, Li)), nl * w / lightPdf);
+                            }
+                        }
+                        float bp = 0.0f;
+                        int st = 0;
+                        float3 wi2 = mk(0, 0, 1);
+                        float3 f2 = bsdf_sample(b, wo, wi2, us.x, us.y, bp, mask, st);
+                        f2 = mul(f2, fabsf(dot(wi2, b.n)));
+                        if (!black(f2) && bp > 0.0f) {
+                            float lp2 = is_env ? env_pdf(sc.env, wi2) : area_pdf(sc.lights[li], p, ng, wi2);
+                            if (lp2 != 0.0f) {
+                                float w = power_heuristic(bp, lp2);
+                                pushM = true;
+                                m_o = offset_origin(p, ng, wi2);
+                                m_d = wi2;
+                                m_light = li;
+                                m_c = mul(cmul(beta, f2), nl * w / bp);
+                            }
+                        }
+                    }
+                    // Continuation (path.cc:83-94) and Russian roulette (path.cc:116-121)
+                    float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
+                    float pdf = 0.0f;
+                    int st = 0;
+                    float3 wi = mk(0, 0, 1);
+                    float3 f = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf, BX_ALL, st);
+                    if (!black(f) && pdf != 0.0f) {
+                        float3 nb = cmul(beta, mul(f, fabsf(dot(wi, b.n)) / pdf));
+                        bool cont = true;
+                        if (depth > rr_depth) {
+                            float q = fminf(0.95f, gray(nb));
+                            if (sample_1d(kind, seed, pixel, sample, dim_base + 7) > q) cont = false;
+                            else nb = mul(nb, 1.0f / q);
+                        }
+                        if (cont) {
+                            pushA = true;
+                            nA_o = offset_origin(p, ng, wi);
+                            nA_d = wi;
+                            nA_beta = nb;
+                            nA_flags = (flags & ~0x7fu) | (uint32_t)(depth + 1) | ((st & BX_SPECULAR) ? (1u << 6) : 0u);
+                        }
+                    }
+                }
+            }
+            if (!black(Ladd)) {
+                float4 Lv = rb.path_L[path];
+                rb.path_L[path] = make_float4(Lv.x + Ladd.x, Lv.y + Ladd.y, Lv.z + Ladd.z, Lv.w);
+            }
+        }
+        const uint32_t sa = wave_append(pushA, &ps->countA[depth + 1]);
+        const uint32_t ss = wave_append(pushS, &ps->countS[depth]);
+        const uint32_t sm = wave_append(pushM, &ps->countM[depth]);
+        if (pushA) {
+            rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
+            rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
+            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+        }
+        if (pushS) {
+            rb.qs_o[ss] = make_float4(s_o.x, s_o.y, s_o.z, __uint_as_float(path));
+            rb.qs_d[ss] = make_float4(s_d.x, s_d.y, s_d.z, s_tmax);
+            rb.qs_c[ss] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
+        }
+        if (pushM) {
+            rb.qm_o[sm] = make_float4(m_o.x, m_o.y, m_o.z, __uint_as_float(path));
+            rb.qm_d[sm] = make_float4(m_d.x, m_d.y, m_d.z, __int_as_float(m_light));
+            rb.qm_c[sm] = make_float4(m_c.x, m_c.y, m_c.z, 0.0f);
+        }
+    }
+}
+
+// ---- shadow: NEE light-sample occlusion (visibility_tester.cc:21-24) ------------------------
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, int depth) {
+    __shared__ uint32_t lds[S * BLOCK];
+    PassState* ps = rb.ps;
+    const uint32_t count = ps->countS[depth];
+    if (count == 0) return;
+    uint32_t* col = lds + threadIdx.x;
+    uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
+    for (;;) {
+        uint32_t base = fetch_work(&ps->workS[depth]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        bool active = i < count;
+        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);
+        if (active) {
+            O = rb.qs_o[i];
+            D = rb.qs_d[i];
+        }
+        float t, u, v;
+        Counters cnt = {0, 0};
+        int32_t k = traverse<BLOCK, S, true, false>(sc, xyz(O), xyz(D), D.w, t, u, v, col, spill, active, cnt);
+        if (active && k < 0) {
+            uint32_t path = __float_as_uint(O.w);
+            float4 c = rb.qs_c[i];
+            float4 Lv = rb.path_L[path];
+            rb.path_L[path] = make_float4(Lv.x + c.x, Lv.y + c.y, Lv.z + c.z, Lv.w);
+        }
+    }
+}
+
+// ---- MIS: NEE BSDF-sample ray; counts if it reaches the chosen light (mis.cc:114-133) ----------
+template <int BLOCK, int S>
+__global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int depth) {
+    __shared__ uint32_t lds[S * BLOCK];
+    PassState* ps = rb.ps;
+    const uint32_t count = ps->countM[depth];
+    if (count == 0) return;
+    uint32_t* col = lds + threadIdx.x;
+    uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
+    for (;;) {
+        uint32_t base = fetch_work(&ps->workM[depth]);
+        if (base >= count) break;
+        uint32_t i = base + __lane_id();
+        bool active = i < count;
+        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);
+        if (active) {
+            O = rb.qm_o[i];
+            D = rb.qm_d[i];
+        }
+        float t, u, v;
+        Counters cnt = {0, 0};
+        int32_t k = traverse<BLOCK, S, false, false>(sc, xyz(O), xyz(D), kINFTY, t, u, v, col, spill, active, cnt);
+        if (active) {
+            int li = __float_as_int(D.w);
+            float3 Le = mk(0, 0, 0);
+            if (k >= 0) {
+                if (li < sc.num_areas && sc.tri_ids[k].y == li && dot(xyz(sc.tri_ng[k]), neg(xyz(D))) > 0.0f)
+                    Le = xyz(sc.lights[li].L);
+            } else if (li >= sc.num_areas) {
+                Le = env_Le(sc.env, xyz(D));
+            }
+            if (!black(Le)) {
+                uint32_t path = __float_as_uint(O.w);
+                float4 c = rb.qm_c[i];
+                float4 Lv = rb.path_L[path];
+                rb.path_L[path] = make_float4(Lv.x + c.x * Le.x, Lv.y + c.y * Le.y, Lv.z + c.z * Le.z, Lv.w);
+            }
+        }
+    }
+}
+
+// ---- film: Kahan-compensated accumulation (film.cc:65-74) -------------------------------------------
+__device__ __forceinline__ void kahan(float& s, float& c, float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+__global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg, RenderBufs rb, int max_depth) {
+    PassState* ps = rb.ps;
+    const uint32_t n_pix = ps->n_pix, k = ps->k_samples, pixel_base = ps->pixel_base;
+    const int W = cfg->W, H = cfg->H;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_pix; j += gridDim.x * blockDim.x) {
+        int x, y;
+        swizzle(pixel_base + j, W, H, x, y);
+        uint32_t f = (uint32_t)(y * W + (W - 1 - x));
+        float4 s = rb.film_sum[f], c = rb.film_c[f], q = rb.film_sq[f], qc = rb.film_sqc[f];
+        for (uint32_t si = 0; si < k; si++) {
+            float4 L = rb.path_L[si * n_pix + j];
+            kahan(s.x, c.x, L.x);
+            kahan(s.y, c.y, L.y);
+            kahan(s.z, c.z, L.z);
+            kahan(q.x, qc.x, L.x * L.x);
+            kahan(q.y, qc.y, L.y * L.y);
+            kahan(q.z, qc.z, L.z * L.z);
+        }
+        s.w += (float)k;
+        rb.film_sum[f] = s;
+        rb.film_c[f] = c;
+        rb.film_sq[f] = q;
+        rb.film_sqc[f] = qc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !ps->done) {
+        unsigned long long e = 0, sh = 0, mi = 0;
+        for (int d = 0; d <= max_depth; d++) {
+            e += ps->countA[d];
+            sh += ps->countS[d];
+            mi += ps->countM[d];
+        }
+        ps->total_paths += ps->n;
+        ps->total_extend += e;
+        ps->total_shadow += sh;
+        ps->total_mis += mi;
+        ps->total_samples += k;
+    }
+}
+
+__global__ void k_clear_film(RenderBufs rb, uint32_t npix) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        float4 z = make_float4(0, 0, 0, 0);
+        rb.film_sum[i] = z;
+        rb.film_c[i] = z;
+        rb.film_sq[i] = z;
+        rb.film_sqc[i] = z;
+    }
+}
+
+// relMSE in float64: mean over pixels and channels of (I - R)^2 / (R^2 + eps).
+__global__ __launch_bounds__(256) void k_relmse(RenderBufs rb, uint32_t npix, const double* ref, double eps, double* partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        float4 s = rb.film_sum[i];
+        double inv = 1.0 / ((double)s.w + 1.0e-12);
+        double I[3] = {(double)s.x * inv, (double)s.y * inv, (double)s.z * inv};
+        for (int c = 0; c < 3; c++) {
+            double R = ref[3 * (size_t)i + c];
+            double dd = I[c] - R;
+            acc += dd * dd / (R * R + eps);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_reduce_sum(const double* partial, int n, double* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+constexpr int kStackSlots = 32;
+
+__global__ void k_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
+                          uint32_t* out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = sample_u32(kind, seed, px[i], sm[i], dm[i]);
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+}  // namespace
+
+int zero_u32(uint32_t* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, p, n);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
+                   uint32_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sampler, dim3(256), dim3(256), 0, st, kind, seed, n, px, sm, dm, out);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int trace_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots>, 128, 0) != hipSuccess) return 4;
+    return nb;
+}
+
+// ---- launchers ---------------------------------------------------------------------------------------
+int launch_trace(const SceneDev& s, const TraceBufs& b, uint32_t* work_counter, uint32_t* spill, int spill_cap, int grid,
+                 hipStream_t st) {
+    constexpr int B = 128;
+    if (b.any_hit)
+        hipLaunchKernelGGL((k_trace<B, kStackSlots, true>), dim3(grid), dim3(B), 0, st, s, b, work_counter, spill, spill_cap);
+    else
+        hipLaunchKernelGGL((k_trace<B, kStackSlots, false>), dim3(grid), dim3(B), 0, st, s, b, work_counter, spill, spill_cap);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_pass, dim3(1), dim3(64), 0, st, ps, index);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_film, dim3(1024), dim3(256), 0, st, b, npix);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial, int nblocks,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_relmse, dim3(nblocks), dim3(256), 0, st, b, cfg.npix, ref, eps, out_partial);
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, st, out_partial, nblocks, out_partial + nblocks);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
+                hipEvent_t* ev_ext, int n_ev_ext) {
+    constexpr int B = 128;
+    hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(256), 0, st, b.cfg, b.ps);
+    hipLaunchKernelGGL(k_camera, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
+    for (int d = 0; d <= cfg.max_depth; d++) {
+        if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d], st));
+        hipLaunchKernelGGL((k_extend<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+        if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
+        hipLaunchKernelGGL((k_shade<B>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+        if (d < cfg.max_depth) {
+            hipLaunchKernelGGL((k_shadow<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_mis<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+        }
+    }
+    hipLaunchKernelGGL(k_film, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+}  // namespace spk

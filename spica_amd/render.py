@@ -1,0 +1,149 @@
+"""Python handles over the C ABI (include/spk.h).
+
+`GpuScene` owns an spk_scene (BVH + geometry + materials + lights in HBM) and
+answers batched ray queries on device tensors; `GpuRender` owns an spk_render
+(path queues + film) and accumulates samples per pixel.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+_initialised = {}
+
+
+def init(device=0):
+    lib = _abi.load()
+    if _initialised.get(device):
+        return lib
+    _abi.check(lib.spk_init(device), "spk_init")
+    _initialised[device] = True
+    return lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+class GpuScene:
+    def __init__(self, scene, device=0):
+        self.lib = init(device)
+        self.scene = scene
+        self._desc = scene.desc()
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.spk_scene_create(ctypes.byref(self._desc), ctypes.byref(h)), "spk_scene_create")
+        self.handle = h
+
+    def stats(self):
+        st = _abi.SpkSceneStats()
+        _abi.check(self.lib.spk_scene_stats_get(self.handle, ctypes.byref(st)), "spk_scene_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def trace(self, org, dir, tmax=None, any_hit=False, stream=None):
+        """org/dir: float32 [n, 3] device tensors; returns (prim int32 [n], t float32 [n] or None)."""
+        import torch
+
+        assert org.is_cuda and dir.is_cuda and org.dtype == torch.float32 and dir.dtype == torch.float32
+        org = org.contiguous()
+        dir = dir.contiguous()
+        n = org.shape[0]
+        assert org.shape == (n, 3) and dir.shape == (n, 3)
+        if tmax is not None:
+            tmax = tmax.contiguous().float()
+            assert tmax.shape == (n,)
+        prim = torch.empty(n, dtype=torch.int32, device=org.device)
+        t = None if any_hit else torch.empty(n, dtype=torch.float32, device=org.device)
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        _abi.check(self.lib.spk_trace(self.handle, _ptr(org), _ptr(dir), _ptr(tmax), n, 1 if any_hit else 0, _ptr(prim),
+                                      _ptr(t), ctypes.c_void_p(st)), "spk_trace")
+        return prim, t
+
+    def close(self):
+        if self.handle:
+            self.lib.spk_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuRender:
+    def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False):
+        self.lib = gscene.lib
+        self.gscene = gscene
+        self.camera = camera
+        self._cam = camera.desc()
+        self._rd = render_desc
+        self._opts = _abi.SpkRenderOpts()
+        self._opts.queue_size = int(queue_size)
+        self._opts.profile = 1 if profile else 0
+        self._opts.use_graph = 1 if use_graph else 0
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.spk_render_create(gscene.handle, ctypes.byref(self._cam), ctypes.byref(self._rd),
+                                              ctypes.byref(self._opts), ctypes.byref(h)), "spk_render_create")
+        self.handle = h
+
+    def run(self, n_spp, sync=True):
+        _abi.check(self.lib.spk_render_run(self.handle, int(n_spp), 1 if sync else 0), "spk_render_run")
+
+    def sync(self):
+        _abi.check(self.lib.spk_render_sync(self.handle), "spk_render_sync")
+
+    def film(self):
+        """(sum [h, w, 3], sumsq [h, w, 3], count [h, w]) as float64, film order (x mirrored as spica)."""
+        w, h = self.camera.width, self.camera.height
+        s = np.zeros((h, w, 3))
+        q = np.zeros((h, w, 3))
+        c = np.zeros((h, w))
+        _abi.check(self.lib.spk_render_film(self.handle, s.ctypes.data_as(_abi.c_d_p), q.ctypes.data_as(_abi.c_d_p),
+                                            c.ctypes.data_as(_abi.c_d_p)), "spk_render_film")
+        return s, q, c
+
+    def image(self):
+        w, h = self.camera.width, self.camera.height
+        img = np.zeros((h, w, 3), dtype=np.float32)
+        _abi.check(self.lib.spk_render_image(self.handle, img.ctypes.data_as(_abi.c_f_p)), "spk_render_image")
+        return img
+
+    def relmse(self, ref, eps=1e-2):
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        out = ctypes.c_double()
+        _abi.check(self.lib.spk_render_relmse(self.handle, ref.ctypes.data_as(_abi.c_d_p), eps, ctypes.byref(out)),
+                   "spk_render_relmse")
+        return out.value
+
+    def stats(self):
+        st = _abi.SpkRenderStats()
+        _abi.check(self.lib.spk_render_stats_get(self.handle, ctypes.byref(st)), "spk_render_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.spk_render_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sampler_u32(kind, seed, pixel, sample, dim, device=False):
+    """Raw 32-bit sample words of the counter-based streams (host or GPU evaluation)."""
+    lib = _abi.load()
+    pixel = np.ascontiguousarray(pixel, dtype=np.uint32)
+    sample = np.ascontiguousarray(sample, dtype=np.uint32)
+    dim = np.ascontiguousarray(dim, dtype=np.uint32)
+    n = len(pixel)
+    out = np.zeros(n, dtype=np.uint32)
+    if device:
+        init(0)
+    _abi.check(lib.spk_sampler_u32(kind, seed, n, pixel.ctypes.data_as(_abi.c_u32_p), sample.ctypes.data_as(_abi.c_u32_p),
+                                   dim.ctypes.data_as(_abi.c_u32_p), out.ctypes.data_as(_abi.c_u32_p), 1 if device else 0),
+               "spk_sampler_u32")
+    return out
