@@ -1,0 +1,178 @@
+"""Benchmark: plain wavefront path tracing of the 512-sphere field on one MI355X.
+
+Workload: seed-1 sphere field, 512 spheres x 640 triangles
+on a ground quad (327,686 triangles), two quad lights, 1920x1080, maxDepth 16,
+Russian roulette after bounce 3, Sobol sampler.  One step = one sample per pixel
+over the whole frame (2,073,600 paths), i.e. one pass of the hot path; the full
+256-spp render is 256 steps.
+
+Printed (one JSON line): total traced rays per second (camera + continuation +
+shadow + MIS rays, all traversals), the closest-hit traversal kernel against the
+HBM roofline, and spica's CPU path (the FP64 C restatement, the reference's
+estimator and BVH) timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to relMSE 1e-3 (guided vs plain PT, 1 GPU); Mrays/s + roofline frac"
+HBM_PEAK_GBS = 8000.0
+B_RAY_IO = 32 + 16          # ray in (o, d, tmax, id) + hit out (t, u, v, prim)
+B_NODE = 64                 # one BVH node: two child boxes + links
+B_TRI = 48                  # one triangle: v0 | id, e1, e2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(profile_dir):
+    """HBM bytes per closest-hit launch from committed rocprofv3 PMC summaries, or None."""
+    path = os.path.join(profile_dir, "extend_c1_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(w, threads, budget_s):
+    """The CPU checker (spica's estimator, FP64, spica's BVH) on a bounded pixel sample."""
+    import numpy as np
+
+    from oracle.oracle import Oracle
+
+    orc = Oracle()
+    t0 = time.perf_counter()
+    sc = orc.scene(w.scene)
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    npix = w.camera.width * w.camera.height
+    # calibrate on a small sample, then run the sample that fits the budget
+    n = 2000
+    rays = paths = 0
+    elapsed = 0.0
+    while elapsed < budget_s:
+        subset = rng.choice(npix, size=n, replace=False).astype(np.int32)
+        rd = w.render_desc(spp=1, first_sample=int(rng.integers(0, 1 << 20)))
+        t0 = time.perf_counter()
+        _, _, _, st = sc.render(w.camera, rd, threads=threads, subset=subset)
+        dt = time.perf_counter() - t0
+        elapsed += dt
+        rays += int(st[0])
+        paths += n
+        n = int(min(200000, max(2000, n * max(1.0, 0.5 * (budget_s - elapsed) / max(dt, 1e-3)))))
+    return dict(value=rays / elapsed / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                sample=f"{paths} random pixels of the 1920x1080 frame at 1 spp ({rays} rays, {elapsed:.1f} s); "
+                       f"spica-style BVH build {build_s:.1f} s excluded",
+                mpaths_per_s=paths / elapsed / 1e6)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--queue", type=int, default=1 << 21)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "paths shard by pixel/sample range with one film reduce, not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from spica_amd import configs, render
+
+    log("bench: building C1 scene")
+    w = configs.c1_spheres()
+    gs = render.GpuScene(w.scene)
+    sst = gs.stats()
+    log(f"bench: {sst['num_tris']} tris, {sst['num_nodes']} nodes, BVH build {sst['build_ms']:.0f} ms")
+
+    # Counting pass (outside the timed region): node fetches / triangle tests per closest-hit ray.
+    gc = render.GpuRender(gs, w.camera, w.render_desc(spp=4), queue_size=args.queue, count_visits=True)
+    gc.run(4)
+    cst = gc.stats()
+    gc.close()
+    v_node = cst["count_nodes"] / max(1, cst["count_rays"])
+    v_tri = cst["count_tris"] / max(1, cst["count_rays"])
+    b_ray = B_RAY_IO + B_NODE * v_node + B_TRI * v_tri
+    log(f"bench: V_node {v_node:.2f}  V_tri {v_tri:.2f}  B_ray {b_ray:.0f} B")
+
+    gr = render.GpuRender(gs, w.camera, w.render_desc(spp=args.warmup + args.steps), queue_size=args.queue,
+                          profile=True)
+    gr.run(args.warmup)
+    st0 = gr.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gr.run(args.steps, sync=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st1 = gr.stats()
+    rays = (st1["rays_extend"] - st0["rays_extend"]) + (st1["rays_shadow"] - st0["rays_shadow"]) + \
+           (st1["rays_mis"] - st0["rays_mis"])
+    paths = st1["paths"] - st0["paths"]
+    ext_ms = st1["extend_ms"] - st0["extend_ms"]
+    ext_launches = st1["extend_launches"] - st0["extend_launches"]
+    ext_rays = st1["extend_rays_timed"] - st0["extend_rays_timed"]
+    value = rays / dt / 1e6
+    avg_launch_s = ext_ms / 1e3 / max(1, ext_launches)
+    bytes_per_launch = b_ray * ext_rays / max(1, ext_launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    log(f"bench: {args.steps} steps in {dt*1e3:.1f} ms: {value:.1f} Mrays/s, {paths/dt/1e6:.1f} Mpaths/s; "
+        f"closest-hit {ext_rays} rays in {ext_ms:.1f} ms over {ext_launches} launches")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: CPU baseline on {args.cpu_threads} threads for ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(w, args.cpu_threads, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C1 sphere field: 512 spheres x 640 tris + ground + 2 quad lights (327686 tris), "
+                               "1920x1080, 1 spp per step, maxDepth 16, RR after 3, Sobol seed 1",
+                   "queue": args.queue},
+        "mpaths_per_s": round(paths / dt / 1e6, 3),
+        "rays_per_path": round(rays / max(1, paths), 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_extend (closest-hit traversal)",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(os.path.join(ROOT, "profiles")),
+            "b_ray": round(b_ray, 1),
+            "v_node": round(v_node, 3),
+            "v_tri": round(v_tri, 3),
+            "avg_launch_us": round(avg_launch_s * 1e6, 2),
+            "rays_per_launch": round(ext_rays / max(1, ext_launches), 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    gr.close()
+    gs.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,7 @@ typedef struct spk_render_opts {
     int32_t block_size;     /* 64, 128 or 256 (0 = auto) */
     int32_t use_graph;      /* capture each pass into a hipGraph and replay it */
     int32_t profile;        /* time every traversal launch with HIP events */
+    int32_t count_visits;   /* count node fetches / triangle tests of the closest-hit stage */
 } spk_render_opts;
 
 /* Creates a render of `scene` through `camera`: film, path queues and stream.
@@ -123,6 +124,9 @@ typedef struct spk_render_stats {
     double shadow_ms;
     int64_t shadow_launches;
     double shade_ms;
+    int64_t count_nodes;    /* count_visits = 1: node fetches of the closest-hit stage */
+    int64_t count_tris;     /* count_visits = 1: triangle tests of the closest-hit stage */
+    int64_t count_rays;     /* count_visits = 1: closest-hit rays those counts cover */
 } spk_render_stats;
 int spk_render_stats_get(spk_render* render, spk_render_stats* out);
 

@@ -106,6 +106,7 @@ class SpkRenderOpts(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("profile", ctypes.c_int32),
+        ("count_visits", ctypes.c_int32),
     ]
 
 
@@ -123,6 +124,9 @@ class SpkRenderStats(ctypes.Structure):
         ("shadow_ms", ctypes.c_double),
         ("shadow_launches", ctypes.c_int64),
         ("shade_ms", ctypes.c_double),
+        ("count_nodes", ctypes.c_int64),
+        ("count_tris", ctypes.c_int64),
+        ("count_rays", ctypes.c_int64),
     ]
 
 

@@ -754,7 +754,12 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
     if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
     int64_t passes = c.chunks > 1 ? (int64_t)n_spp * c.chunks : ((int64_t)n_spp + c.kmax - 1) / c.kmax;
     int nev = 2 * (c.max_depth + 1);
+    int64_t ext_before = 0;
     if (r->opts.profile) {
+        PassState ps;
+        SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+        SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
+        ext_before = (int64_t)ps.total_extend;
         size_t need = (size_t)passes * nev;
         while (r->events.size() < need) {
             hipEvent_t e;
@@ -764,11 +769,15 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
     }
     for (int64_t p = 0; p < passes; p++) {
         hipEvent_t* ev = r->opts.profile ? &r->events[(size_t)p * nev] : nullptr;
-        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0))) return rc;
+        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0, r->opts.count_visits != 0)))
+            return rc;
     }
     r->samples_done += n_spp;
     if (r->opts.profile) {
         SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+        PassState ps;
+        SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
+        r->extend_rays_timed += (int64_t)ps.total_extend - ext_before;
         for (int64_t p = 0; p < passes; p++)
             for (int d = 0; d <= c.max_depth; d++) {
                 float ms = 0.0f;
@@ -866,6 +875,10 @@ int spk_render_stats_get(spk_render* r, spk_render_stats* out) {
     out->passes = (int64_t)ps.total_samples;
     out->extend_ms = r->extend_ms;
     out->extend_launches = r->extend_launches;
+    out->extend_rays_timed = r->extend_rays_timed;
+    out->count_nodes = (int64_t)ps.count_nodes;
+    out->count_tris = (int64_t)ps.count_tris;
+    out->count_rays = (int64_t)ps.count_rays;
     return SPK_OK;
 }
 

@@ -114,6 +114,9 @@ struct PassState {
     unsigned long long total_shadow;
     unsigned long long total_mis;
     unsigned long long total_samples;
+    unsigned long long count_nodes;  // extend traversal node fetches (counting passes)
+    unsigned long long count_tris;   // extend traversal triangle tests (counting passes)
+    unsigned long long count_rays;   // extend rays traced in counting passes
 };
 
 }  // namespace spk

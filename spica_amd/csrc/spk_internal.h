@@ -105,7 +105,7 @@ struct LaunchShape {
 };
 
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls,
-                hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext);
+                hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count);
 int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st);
 int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st);
 int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial,

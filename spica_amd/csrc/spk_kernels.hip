@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cf
 }
 
 // ---- extend: closest hit over the extend queue -------------------------------------------------
-template <int BLOCK, int S>
+template <int BLOCK, int S, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, int depth) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
@@ -416,8 +416,13 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, in
         }
         float t, u, v;
         Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, false, false>(sc, o, d, kINFTY, t, u, v, col, spill, active, cnt);
+        int32_t k = traverse<BLOCK, S, false, COUNT>(sc, o, d, kINFTY, t, u, v, col, spill, active, cnt);
         if (active) rb.hit[i] = make_float4(t, u, v, __int_as_float(k));
+        if (COUNT && active) {
+            atomicAdd(&ps->count_nodes, (unsigned long long)cnt.nodes);
+            atomicAdd(&ps->count_tris, (unsigned long long)cnt.tris);
+            atomicAdd(&ps->count_rays, 1ull);
+        }
     }
 }
 
@@ -784,14 +789,14 @@ __device__ __forceinline__ int find_interval(const float* cdf, int size, float v
 }
 // envmap.cc:60-79
 __device__ float3 env_sample(const EnvDev& e, float u0, float u1, float3& dir, float& pdf, float& uu, float& vv) {
-    int vo = find_interval(e.marg_cdf, e.h + 1, u1);
+    int vo = min(max(find_interval(e.marg_cdf, e.h + 1, u1), 0), e.h - 1);
     float c0 = e.marg_cdf[vo], c1 = e.marg_cdf[vo + 1];
     float du = u1 - c0;
     if (c1 - c0 > 0.0f) du /= (c1 - c0);
     float p1 = e.marg_pdf[vo];
     vv = ((float)vo + du) / (float)e.h;
     const float* rc = e.cond_cdf + (size_t)vo * (e.w + 1);
-    int uo = find_interval(rc, e.w + 1, u0);
+    int uo = min(max(find_interval(rc, e.w + 1, u0), 0), e.w - 1);
     float d0 = rc[uo], d1 = rc[uo + 1];
     float dv = u0 - d0;
     if (d1 - d0 > 0.0f) dv /= (d1 - d0);
@@ -1213,7 +1218,7 @@ int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const
 
 int trace_blocks_per_cu() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots>, 128, 0) != hipSuccess) return 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots, false>, 128, 0) != hipSuccess) return 4;
     return nb;
 }
 
@@ -1250,13 +1255,16 @@ int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, 
 }
 
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
-                hipEvent_t* ev_ext, int n_ev_ext) {
+                hipEvent_t* ev_ext, int n_ev_ext, bool count) {
     constexpr int B = 128;
     hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(256), 0, st, b.cfg, b.ps);
     hipLaunchKernelGGL(k_camera, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
     for (int d = 0; d <= cfg.max_depth; d++) {
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d], st));
-        hipLaunchKernelGGL((k_extend<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+        if (count)
+            hipLaunchKernelGGL((k_extend<B, kStackSlots, true>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+        else
+            hipLaunchKernelGGL((k_extend<B, kStackSlots, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
         hipLaunchKernelGGL((k_shade<B>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
         if (d < cfg.max_depth) {

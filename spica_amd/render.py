@@ -72,7 +72,7 @@ class GpuScene:
 
 
 class GpuRender:
-    def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False):
+    def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False, count_visits=False):
         self.lib = gscene.lib
         self.gscene = gscene
         self.camera = camera
@@ -82,6 +82,7 @@ class GpuRender:
         self._opts.queue_size = int(queue_size)
         self._opts.profile = 1 if profile else 0
         self._opts.use_graph = 1 if use_graph else 0
+        self._opts.count_visits = 1 if count_visits else 0
         h = ctypes.c_void_p()
         _abi.check(self.lib.spk_render_create(gscene.handle, ctypes.byref(self._cam), ctypes.byref(self._rd),
                                               ctypes.byref(self._opts), ctypes.byref(h)), "spk_render_create")

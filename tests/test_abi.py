@@ -1,0 +1,55 @@
+"""The C ABI library: loads without a GPU, exports every entry point include/spk.h
+declares, and rejects malformed descriptors with an error status (no abort)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from spica_amd import _abi, configs
+from tests.conftest import ROOT
+
+
+def _declared():
+    names = set()
+    for h in ("spk.h",):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\*?\s+\*?(spk_\w+)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_exports_every_declared_symbol():
+    lib = _abi.load()
+    declared = _declared()
+    assert "spk_trace" in declared and "spk_render_run" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_abi.SIGNATURES) == declared
+
+
+def test_version_and_error_string():
+    lib = _abi.load()
+    assert b"gfx950" in lib.spk_version()
+    assert lib.spk_last_error() is not None
+
+
+def test_bad_descriptors_are_rejected():
+    lib = _abi.load()
+    w = configs.cornell_box()
+    d = w.scene.desc()
+    bad = np.full(w.scene.num_tris, 99, dtype=np.int32)
+    d.mat_ids = bad.ctypes.data_as(_abi.c_i32_p)
+    h = ctypes.c_void_p()
+    rc = lib.spk_scene_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _abi.SPK_ERR_INVALID and b"material" in lib.spk_last_error()
+    d = w.scene.desc()
+    lid = w.scene.freeze()["light_ids"].copy()
+    lid[0] = 0  # triangle 0 claims light 0, which names another triangle
+    d.light_ids = lid.ctypes.data_as(_abi.c_i32_p)
+    rc = lib.spk_scene_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _abi.SPK_ERR_INVALID and b"light" in lib.spk_last_error()
+    assert lib.spk_scene_create(None, ctypes.byref(h)) == _abi.SPK_ERR_INVALID
+    assert lib.spk_render_run(None, 1, 0) == _abi.SPK_ERR_INVALID
+    assert lib.spk_sampler_u32(7, 0, 0, None, None, None, None, 0) == _abi.SPK_ERR_INVALID

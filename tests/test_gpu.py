@@ -1,0 +1,211 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU checker.
+
+Closest hits: primitive ids equal the float64 path's on seeded ray sets, except
+mismatches classified as near-ties (both triangles hit, |dt| <= 1e-6 t) or edge
+cases (a barycentric coordinate within 1e-4 of an edge, where float32 and float64
+legitimately disagree); no other class is allowed.
+Images: per-pixel means against the checker's independent render with a Welch z
+per pixel and channel.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from spica_amd import configs, meshio
+from spica_amd.scene import Scene, diffuse
+from tests.conftest import GOLDEN
+from tests.helpers import classify_mismatches, furnace_room, ray_set, small_mixed_scene, zstats
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace_parity(gpu, oracle, scene, org, dirv, max_mismatch_frac):
+    import torch
+
+    gs = gpu.GpuScene(scene)
+    prim, t = gs.trace(torch.from_numpy(org).cuda(), torch.from_numpy(dirv).cuda())
+    pg, tg = prim.cpu().numpy(), t.cpu().numpy()
+    pc, tc, _ = oracle.scene(scene).trace(org.astype(np.float64), dirv.astype(np.float64), threads=16)
+    frac = float(np.mean(pg != pc))
+    verts = scene.freeze()["verts"]
+    cls = classify_mismatches(verts, org, dirv, pg, tg, pc, tc)
+    print(f"rays {len(org)} hits {(pc >= 0).sum()} mismatches {int((pg != pc).sum())} ({frac:.2e}) {cls}")
+    assert cls["other"] == 0, cls["other_idx"][:10]
+    assert frac <= max_mismatch_frac
+    hit = (pg >= 0) & (pg == pc)
+    assert np.allclose(tg[hit], tc[hit], rtol=2e-5, atol=1e-5)
+    gs.close()
+    return pg, tg
+
+
+def test_sampler_streams_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(0)
+    n = 200000
+    px = rng.integers(0, 2 ** 31, n, dtype=np.uint32)
+    sm = rng.integers(0, 65536, n, dtype=np.uint32)
+    dm = rng.integers(0, 300, n, dtype=np.uint32)
+    for kind in (0, 1):
+        assert np.array_equal(gpu.sampler_u32(kind, 7, px, sm, dm, device=True), oracle.sampler_u32(kind, 7, px, sm, dm))
+
+
+def test_box_ply_known_answer(gpu):
+    import torch
+
+    tris, _ = meshio.load_ply(os.path.join(GOLDEN, "box.ply"))
+    s = Scene()
+    s.add_triangles(tris, diffuse((0.5, 0.5, 0.5)))
+    gs = gpu.GpuScene(s)
+    prim, t = gs.trace(torch.tensor([[0.0, 0.0, 100.0]]).cuda(), torch.tensor([[0.0, 0.0, -1.0]]).cuda())
+    assert int(prim[0]) >= 0 and float(t[0]) == 99.5
+    v = tris[int(prim[0])].astype(np.float64)
+    n = np.cross(v[1] - v[0], v[2] - v[0])
+    assert np.allclose(n / np.linalg.norm(n), [0, 0, 1])
+
+
+def test_trace_parity_cornell(gpu, oracle):
+    w = configs.cornell_box()
+    org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 300000, seed=1)
+    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-2)
+
+
+@pytest.mark.parametrize("mesh", ["bunny", "kitten"])
+def test_trace_parity_meshes(gpu, oracle, mesh):
+    tris, _ = meshio.load_ply(os.path.join(GOLDEN, f"{mesh}.ply"))
+    s = Scene()
+    s.add_triangles(tris, diffuse((0.5, 0.5, 0.5)))
+    rng = np.random.default_rng(2)
+    lo, hi = s.bounds()
+    n = 1000000
+    org = rng.uniform(lo - 0.25 * (hi - lo), hi + 0.25 * (hi - lo), size=(n, 3)).astype(np.float32)
+    dirv = (rng.uniform(lo, hi, size=(n, 3)) - org).astype(np.float32)
+    _trace_parity(gpu, oracle, s, org, dirv, 1e-3)
+
+
+def test_trace_parity_c1_10m(gpu, oracle):
+    """10M seeded rays (uniform / camera / first bounce) on the 327,686-triangle sphere field."""
+    w = configs.c1_spheres()
+    org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 10_000_000, seed=1)
+    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-4)
+
+
+def test_any_hit_matches_closest(gpu):
+    import torch
+
+    w = configs.c1_spheres(n_spheres=64)
+    gs = gpu.GpuScene(w.scene)
+    rng = np.random.default_rng(3)
+    lo, hi = w.scene.bounds()
+    n = 500000
+    o = torch.from_numpy(rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)).cuda()
+    d = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    prim, t = gs.trace(o, d)
+    occ, _ = gs.trace(o, d, any_hit=True)
+    assert torch.equal(occ == 1, prim >= 0)
+    # segment just short of the closest hit is unoccluded for almost every ray
+    hit = prim >= 0
+    occ2, _ = gs.trace(o[hit], d[hit], tmax=t[hit] * 0.999, any_hit=True)
+    assert float((occ2 == 1).float().mean()) < 1e-3
+
+
+def _render_gpu(gpu, scene, cam, rd, spp, **kw):
+    gs = gpu.GpuScene(scene)
+    gr = gpu.GpuRender(gs, cam, rd, **kw)
+    gr.run(spp)
+    s, q, c = gr.film()
+    st = gr.stats()
+    gr.close()
+    gs.close()
+    return s.reshape(-1, 3), q.reshape(-1, 3), c.reshape(-1), st
+
+
+def _render_cpu(oracle, scene, cam, rd):
+    s, q, c, _ = oracle.scene(scene).render(cam, rd, threads=16)
+    return s, q, c
+
+
+def _z_report(zs, name):
+    a = np.abs(zs)
+    f3 = float(np.mean(a > 3))
+    print(f"{name}: |z|>3 {f3:.4%}, |z|>4 {np.mean(a > 4):.4%}, max |z| {a.max():.2f}")
+    return f3, float(a.max())
+
+
+@pytest.mark.parametrize("depth", [0, 1, 4])
+def test_furnace_closed_form(gpu, depth):
+    rho, Le = 0.5, 1.0
+    s, cam = furnace_room(rho, Le)
+    rd = configs.Workload("furnace", s, cam, 256, depth, 0, 3).render_desc()
+    sm, sq, cnt, _ = _render_gpu(gpu, s, cam, rd, 256)
+    n = cnt.sum()
+    mean = sm.sum(0) / n
+    var = sq.sum(0) / n - mean ** 2
+    want = Le * sum(rho ** k for k in range(depth + 1))
+    print("furnace", depth, mean, want)
+    assert np.all(np.abs(mean - want) < 5 * np.sqrt(var / n) + 1e-6)
+
+
+def test_cornell_means_within_3_sigma(gpu, oracle):
+    """C0 (256x256, maxDepth 8): GPU 256 spp vs the CPU checker's independent 64 spp."""
+    w = configs.cornell_box()
+    sg, qg, cg, _ = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=256), 256)
+    sc, qc, cc = _render_cpu(oracle, w.scene, w.camera, w.render_desc(spp=64, seed=777))
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    f3, zmax = _z_report(z, "cornell")
+    assert f3 <= 0.005
+    assert abs(mg.mean() - mc.mean()) < 3e-3 * mc.mean()
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_mixed_materials_within_3_sigma(gpu, oracle, sampler):
+    """Every BSDF kind (Lambert, mirror, dielectric, GGX), maxDepth 16."""
+    s, cam = small_mixed_scene()
+    wl = configs.Workload("mixed", s, cam, 2048, 16, sampler, 5)
+    sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 2048)
+    sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, sampler=0, seed=99))
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    f3, zmax = _z_report(z, f"mixed sampler={sampler}")
+    assert f3 <= 0.01
+
+
+def test_envmap_within_3_sigma(gpu, oracle):
+    s, cam = small_mixed_scene(n_spheres=16)
+    h, w = 32, 64
+    v = (np.arange(h) + 0.5) / h
+    sky = np.repeat((0.5 + 1.5 * (1 - v))[:, None], w, axis=1)
+    img = np.stack([sky, sky, 1.2 * sky], axis=-1).astype(np.float32)
+    img[5:7, 20:23] = 400.0
+    s.set_envmap(img, to_world=np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], float), radius=30.0)
+    wl = configs.Workload("env", s, cam, 1024, 8, 0, 5)
+    sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 1024)
+    sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, seed=41))
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    f3, zmax = _z_report(z, "envmap")
+    assert f3 <= 0.01
+
+
+def test_film_is_deterministic(gpu):
+    """Counter-based sampling + in-order Kahan film: bit-identical films across runs and
+    across queue sizes (different pass splits)."""
+    w = configs.cornell_box(width=96, height=64)
+    a = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=16), 16)
+    b = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=16), 16)
+    c = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=16), 16, queue_size=96 * 64 * 3)
+    d = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=16), 16, queue_size=1000)
+    for x in (b, c, d):
+        assert np.array_equal(a[0], x[0]) and np.array_equal(a[1], x[1]) and np.array_equal(a[2], x[2])
+
+
+def test_relmse_kernel_matches_numpy(gpu):
+    w = configs.cornell_box(width=64, height=48)
+    gs = gpu.GpuScene(w.scene)
+    gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=8))
+    gr.run(8)
+    img = gr.image().astype(np.float64)
+    rng = np.random.default_rng(1)
+    ref = img * (1 + 0.1 * rng.normal(size=img.shape)) + 0.01
+    want = np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2))
+    got = gr.relmse(ref, 1e-2)
+    assert abs(got - want) <= 1e-5 * want
+    gr.close()
+    gs.close()
