@@ -237,11 +237,26 @@ __device__ __forceinline__ int32_t traverse(const SceneDev& sc, float3 o, float3
     return hit;
 }
 
-// Persistent-thread work fetch: 64 items per wave per atomic.
-__device__ __forceinline__ uint32_t fetch_work(uint32_t* counter) {
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+// Static work split for the persistent traversal grids.  The hardware deals
+// workgroups round-robin over the 8 XCDs; renumbering so that XCD x owns the
+// logical blocks [x * G/8, (x+1) * G/8) gives each XCD (and its L2) a contiguous
+// slice of the queue, i.e. neighbouring, coherent rays.  Each wave then takes
+// 64-item chunks grid-stride: no shared counter, no atomics.
+constexpr uint32_t kXcds = 8;
+struct WaveSplit {
+    uint32_t first;   // first item of this wave
+    uint32_t stride;  // items between successive chunks of this wave
+};
+template <int BLOCK>
+__device__ __forceinline__ WaveSplit wave_split() {
+    uint32_t g = gridDim.x, b = blockIdx.x;
+    uint32_t logical = b;
+    if (g % kXcds == 0) logical = (b % kXcds) * (g / kXcds) + b / kXcds;
+    uint32_t wave = logical * (BLOCK / 64) + threadIdx.x / 64;
+    WaveSplit s;
+    s.first = wave * 64u;
+    s.stride = g * (uint32_t)BLOCK;
+    return s;
 }
 
 // ---- standalone trace (spk_trace) --------------------------------------------------------
@@ -252,9 +267,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneDev sc, TraceBufs b, uint3
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = spill_all + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * spill_cap;
     const uint64_t n = (uint64_t)b.n;
-    for (;;) {
-        uint64_t base = fetch_work(work);
-        if (base >= n) break;
+    const WaveSplit ws = wave_split<BLOCK>();
+    for (uint64_t base = ws.first; base < n; base += ws.stride) {
         uint64_t i = base + __lane_id();
         bool active = i < n;
         float3 o = mk(0, 0, 1), d = mk(0, 0, 1);
@@ -404,9 +418,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, in
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
     const float4* __restrict__ qo = rb.qa_o[depth & 1];
     const float4* __restrict__ qd = rb.qa_d[depth & 1];
-    for (;;) {
-        uint32_t base = fetch_work(&ps->workA[depth]);
-        if (base >= count) break;
+    const WaveSplit ws = wave_split<BLOCK>();
+    for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
         float3 o = mk(0, 0, 0), d = mk(0, 0, 1);
@@ -1035,9 +1048,8 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
-    for (;;) {
-        uint32_t base = fetch_work(&ps->workS[depth]);
-        if (base >= count) break;
+    const WaveSplit ws = wave_split<BLOCK>();
+    for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
         float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);
@@ -1066,9 +1078,8 @@ __global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int d
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
-    for (;;) {
-        uint32_t base = fetch_work(&ps->workM[depth]);
-        if (base >= count) break;
+    const WaveSplit ws = wave_split<BLOCK>();
+    for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
         float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);

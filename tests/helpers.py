@@ -59,7 +59,8 @@ def ray_set(scene, camera, oracle_scene, n, seed=1):
     ty = np.cross(nrm, tx)
     d3 = loc[:, :1] * tx + loc[:, 1:2] * ty + loc[:, 2:] * nrm
     o3 = p + 1e-3 * nrm
-    o3 = np.where(ok[:, None], o3, o1[: n3 % max(1, n1)] if n1 >= n3 else ow[n2:])
+    # camera rays that escaped stay as they are
+    o3 = np.where(ok[:, None], o3, ow[n2:])
     d3 = np.where(ok[:, None], d3, dw[n2:])
     org = np.concatenate([o1, o2, o3]).astype(np.float32)
     dirv = np.concatenate([d1, d2, d3]).astype(np.float32)
@@ -85,8 +86,10 @@ def tri_intersect64(v, org, dirv):
 def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6, edge=1e-4):
     """Mismatched closest-hit ids, classified:
     near_tie  both triangles hit (float64) at t within rel_t of each other;
-    edge      the disagreement is explained by a hit within `edge` (barycentric) of a
-              triangle edge, either side;
+    edge      the disagreement is explained by a hit within the float32 reach of a
+              triangle edge, either side: barycentric distance below
+              max(edge, 32 * 2^-24 * (|o| + t) / sqrt(2 * area)), i.e. a few float32
+              rounding steps of the hit position measured in triangle sizes;
     other     anything else (a real error)."""
     idx = np.nonzero(p_gpu != p_cpu)[0]
     out = {"near_tie": 0, "edge": 0, "other": 0, "other_idx": []}
@@ -95,23 +98,35 @@ def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6
     o = org[idx].astype(np.float64)
     d = dirv[idx].astype(np.float64)
     v = verts.astype(np.float64)
+    def reach(k, t):
+        tri = v[k]
+        area2 = np.linalg.norm(np.cross(tri[1] - tri[0], tri[2] - tri[0]))
+        return max(edge, 32.0 * 2.0 ** -24 * (np.linalg.norm(o[j]) + abs(t)) / np.sqrt(max(area2, 1e-30)))
+
     for j, i in enumerate(idx):
         pg, pc = int(p_gpu[i]), int(p_cpu[i])
         tg = tc = None
         bary_g = bary_c = None
+        eg = ec = edge
         if pg >= 0:
             t, u, w, _ = tri_intersect64(v[pg][None], o[j][None], d[j][None])
             tg, bary_g = t[0], min(u[0], w[0], 1.0 - u[0] - w[0])
+            eg = reach(pg, tg)
         if pc >= 0:
             t, u, w, _ = tri_intersect64(v[pc][None], o[j][None], d[j][None])
             tc, bary_c = t[0], min(u[0], w[0], 1.0 - u[0] - w[0])
-        if pg >= 0 and pc >= 0 and bary_g > -edge and bary_c > -edge and abs(tg - tc) <= rel_t * max(abs(tc), 1e-30):
+            ec = reach(pc, tc)
+        if pg >= 0 and pc >= 0 and bary_g > -eg and bary_c > -ec and abs(tg - tc) <= rel_t * max(abs(tc), 1e-30):
             out["near_tie"] += 1
-        elif (bary_g is not None and abs(bary_g) < edge) or (bary_c is not None and abs(bary_c) < edge):
+        elif (bary_g is not None and abs(bary_g) < eg) or (bary_c is not None and abs(bary_c) < ec):
             out["edge"] += 1
         else:
             out["other"] += 1
             out["other_idx"].append(int(i))
+            out.setdefault("other_detail", []).append(
+                dict(ray=int(i), gpu=(pg, float(t_gpu[i]), None if bary_g is None else float(bary_g)),
+                     cpu=(pc, float(t_cpu[i]), None if bary_c is None else float(bary_c)),
+                     t_gpu_tri64=None if tg is None else float(tg), t_cpu_tri64=None if tc is None else float(tc)))
     return out
 
 

