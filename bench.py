@@ -76,12 +76,26 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--queue", type=int, default=1 << 21)
+    ap.add_argument("--queue", type=int, default=0, help="paths in flight (0 = autotune)")
+    ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"bench: copied {n * 4} bytes x3")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "paths shard by pixel/sample range with one film reduce, not run here", file=sys.stderr)
@@ -97,8 +111,20 @@ def main():
     sst = gs.stats()
     log(f"bench: {sst['num_tris']} tris, {sst['num_nodes']} nodes, BVH build {sst['build_ms']:.0f} ms")
 
+    # Autotune (outside the timed region): paths in flight and traversal block size.
+    table = []
+    if args.queue == 0 or args.block == 0:
+        from spica_amd import autotune
+
+        q, b, table = autotune.tune(gs, w.camera, w.render_desc(), log=log,
+                                    queues=(args.queue,) if args.queue else autotune.DEFAULT_QUEUES,
+                                    blocks=(args.block,) if args.block else autotune.DEFAULT_BLOCKS)
+        args.queue, args.block = q, b
+    log(f"bench: queue {args.queue} block {args.block}")
+
     # Counting pass (outside the timed region): node fetches / triangle tests per closest-hit ray.
-    gc = render.GpuRender(gs, w.camera, w.render_desc(spp=4), queue_size=args.queue, count_visits=True)
+    gc = render.GpuRender(gs, w.camera, w.render_desc(spp=4), queue_size=args.queue, block_size=args.block,
+                          count_visits=True)
     gc.run(4)
     cst = gc.stats()
     gc.close()
@@ -108,7 +134,7 @@ def main():
     log(f"bench: V_node {v_node:.2f}  V_tri {v_tri:.2f}  B_ray {b_ray:.0f} B")
 
     gr = render.GpuRender(gs, w.camera, w.render_desc(spp=args.warmup + args.steps), queue_size=args.queue,
-                          profile=True)
+                          block_size=args.block, profile=True)
     gr.run(args.warmup)
     st0 = gr.stats()
     torch.cuda.synchronize()
@@ -150,7 +176,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "C1 sphere field: 512 spheres x 640 tris + ground + 2 quad lights (327686 tris), "
                                "1920x1080, 1 spp per step, maxDepth 16, RR after 3, Sobol seed 1",
-                   "queue": args.queue},
+                   "queue": args.queue, "block": args.block, "autotune": table},
         "mpaths_per_s": round(paths / dt / 1e6, 3),
         "rays_per_path": round(rays / max(1, paths), 3),
         "roofline": {

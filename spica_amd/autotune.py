@@ -1,0 +1,79 @@
+"""Autotuner for the wavefront render: paths in flight (queue) and traversal block size.
+
+Deep bounces are latency bound (few long rays per launch), so larger queues amortise
+them over more samples per pass, at the cost of queue memory (~224 B per path).
+The tuner times a few passes per candidate, keeps the fastest by traced rays per
+second and caches the choice per (scene size, film, depth, device).
+"""
+import json
+import os
+import time
+
+DEFAULT_QUEUES = (1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25)
+DEFAULT_BLOCKS = (64, 128, 256)
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "spica_amd", "autotune.json")
+
+
+def _key(gscene, camera, desc):
+    import torch
+
+    st = gscene.stats()
+    return f"{torch.cuda.get_device_name(0)}|tris={st['num_tris']}|{camera.width}x{camera.height}|depth={desc.max_depth}"
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(db, f, indent=1)
+    except OSError:
+        pass
+
+
+def tune(gscene, camera, desc, queues=DEFAULT_QUEUES, blocks=DEFAULT_BLOCKS, spp=8, use_cache=True, log=None):
+    """Returns (queue_size, block_size, table) for this scene/film/depth."""
+    import torch
+
+    from . import render
+
+    key = _key(gscene, camera, desc)
+    db = _load() if use_cache else {}
+    if key in db:
+        e = db[key]
+        return e["queue"], e["block"], e.get("table", [])
+    npix = camera.width * camera.height
+    table = []
+    best = None
+    for q in queues:
+        if q // npix > 2 * spp and q > queues[0]:
+            continue  # more samples per pass than the probe runs: no information
+        for b in blocks:
+            gr = render.GpuRender(gscene, camera, desc, queue_size=q, block_size=b)
+            gr.run(max(1, min(spp, q // npix)))
+            st0 = gr.stats()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gr.run(spp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            st1 = gr.stats()
+            gr.close()
+            rays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+            rate = rays / dt / 1e6
+            table.append(dict(queue=q, block=b, mrays_per_s=round(rate, 1), ms_per_spp=round(dt * 1e3 / spp, 3)))
+            if log:
+                log(f"autotune: queue 2^{q.bit_length() - 1} block {b}: {rate:.1f} Mrays/s")
+            if best is None or rate > best[0]:
+                best = (rate, q, b)
+    db[key] = dict(queue=best[1], block=best[2], table=table)
+    if use_cache:
+        _store(db)
+    return best[1], best[2], table

@@ -49,7 +49,7 @@ void DevBuf::release() {
     bytes = 0;
 }
 
-int trace_blocks_per_cu();
+int trace_blocks_per_cu(int block);
 int zero_u32(uint32_t* p, int n, hipStream_t st);
 int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
                    uint32_t* out, hipStream_t st);
@@ -577,7 +577,7 @@ int spk_trace(spk_scene* s, const float* org, const float* dir, const float* tma
     if (!s->trace_work.p) {
         hipDeviceProp_t prop;
         SPK_CHECK_HIP(hipGetDeviceProperties(&prop, s->device));
-        s->trace_grid = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu());
+        s->trace_grid = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu(128));
         if ((rc = s->trace_work.alloc(64))) return rc;
         if ((rc = s->trace_spill.alloc((size_t)s->trace_grid * 128 * s->spill_cap * 4))) return rc;
     }
@@ -628,7 +628,12 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     if (opts) o = *opts;
     if (o.queue_size <= 0) o.queue_size = int64_t(1) << 21;
     if (o.queue_size < 64) o.queue_size = 64;
-    if (o.queue_size > (int64_t(1) << 24)) o.queue_size = int64_t(1) << 24;
+    if (o.queue_size > (int64_t(1) << 26)) o.queue_size = int64_t(1) << 26;
+    if (o.block_size != 0 && o.block_size != 64 && o.block_size != 128 && o.block_size != 256) {
+        set_error("spk_render_create: block_size %d not one of 64, 128, 256", o.block_size);
+        return SPK_ERR_INVALID;
+    }
+    if (o.block_size == 0) o.block_size = 128;
     r->opts = o;
 
     ConfigDev& c = r->cfg;
@@ -649,7 +654,9 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     c.npix = (uint32_t)cam->width * (uint32_t)cam->height;
     uint32_t qcap = (uint32_t)o.queue_size;
     if (c.npix <= qcap) {
-        c.kmax = qcap / c.npix;
+        // samples per pass; bounded so the per-path sample slot fits the queue flags
+        // (bits 7..31) and the film kernel's per-pixel loop stays short
+        c.kmax = std::min<uint32_t>(qcap / c.npix, 1u << 14);
         c.chunks = 1;
         r->capacity = c.kmax * c.npix;
     } else {
@@ -679,10 +686,10 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
         return rc;
     hipDeviceProp_t prop;
     SPK_CHECK_HIP(hipGetDeviceProperties(&prop, r->device));
-    r->shape.block = 128;
-    r->shape.grid_trace = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu());
+    r->shape.block = o.block_size;
+    r->shape.grid_trace = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu(o.block_size));
     r->shape.grid_stream = prop.multiProcessorCount * 8;
-    if ((rc = r->spill.alloc((size_t)r->shape.grid_trace * 128 * s->spill_cap * 4))) return rc;
+    if ((rc = r->spill.alloc((size_t)r->shape.grid_trace * o.block_size * s->spill_cap * 4))) return rc;
     if ((rc = r->cfg_dev.alloc(sizeof(ConfigDev))) || (rc = r->ps_dev.alloc(sizeof(PassState)))) return rc;
     if ((rc = r->scratch.alloc(4096 * sizeof(double)))) return rc;
     SPK_CHECK_HIP(hipMemset(r->ps_dev.p, 0, sizeof(PassState)));

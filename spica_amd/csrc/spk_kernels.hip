@@ -1227,9 +1227,15 @@ int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const
     return SPK_OK;
 }
 
-int trace_blocks_per_cu() {
+int trace_blocks_per_cu(int block) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots, false>, 128, 0) != hipSuccess) return 4;
+    hipError_t e;
+    switch (block) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<64, kStackSlots, false>, 64, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<256, kStackSlots, false>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots, false>, 128, 0); break;
+    }
+    if (e != hipSuccess || nb < 1) return 2;
     return nb;
 }
 
@@ -1265,9 +1271,9 @@ int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, 
     return SPK_OK;
 }
 
-int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
-                hipEvent_t* ev_ext, int n_ev_ext, bool count) {
-    constexpr int B = 128;
+template <int B>
+static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                         hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count) {
     hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(256), 0, st, b.cfg, b.ps);
     hipLaunchKernelGGL(k_camera, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
     for (int d = 0; d <= cfg.max_depth; d++) {
@@ -1286,6 +1292,15 @@ int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, co
     hipLaunchKernelGGL(k_film, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
+}
+
+int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
+                hipEvent_t* ev_ext, int n_ev_ext, bool count) {
+    switch (ls.block) {
+    case 64: return launch_pass_b<64>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
+    case 256: return launch_pass_b<256>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
+    default: return launch_pass_b<128>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
+    }
 }
 
 }  // namespace spk

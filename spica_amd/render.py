@@ -72,7 +72,8 @@ class GpuScene:
 
 
 class GpuRender:
-    def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False, count_visits=False):
+    def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False, count_visits=False,
+                 block_size=0):
         self.lib = gscene.lib
         self.gscene = gscene
         self.camera = camera
@@ -80,6 +81,7 @@ class GpuRender:
         self._rd = render_desc
         self._opts = _abi.SpkRenderOpts()
         self._opts.queue_size = int(queue_size)
+        self._opts.block_size = int(block_size)
         self._opts.profile = 1 if profile else 0
         self._opts.use_graph = 1 if use_graph else 0
         self._opts.count_visits = 1 if count_visits else 0
