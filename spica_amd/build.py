@@ -31,5 +31,32 @@ def build_lib(verbose=False):
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+PLUGIN = os.path.join(HERE, "libspica_path.so")
+DEMO = os.path.join(HOST, "plugin_demo")
+
+
+def build_plugin(verbose=False):
+    """The spica-compatible "path" plugin (C++ over the C ABI) and its demo driver."""
+    cmds = [
+        ["g++", "-std=c++17", "-O2", "-Wall", "-fPIC", "-shared", os.path.join(HOST, "path_plugin.cpp"),
+         "-L" + HERE, "-lspk", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", PLUGIN + ".tmp"],
+        ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HOST, "plugin_demo.cpp"), "-L" + HERE, "-lspica_path",
+         "-lspk", "-ldl", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", DEMO + ".tmp"],
+    ]
+    for i, cmd in enumerate(cmds):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("g++ failed building the path plugin")
+        if i == 0:
+            os.replace(PLUGIN + ".tmp", PLUGIN)
+    os.replace(DEMO + ".tmp", DEMO)
+    return PLUGIN
+
+
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    print(build_plugin(verbose=True))

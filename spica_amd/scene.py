@@ -180,6 +180,26 @@ class Scene:
         v = self.freeze()["verts"].reshape(-1, 3)
         return v.min(axis=0), v.max(axis=0)
 
+    def save_bin(self, path, camera, spp, max_depth, sampler, seed):
+        """Flat scene + render request for the C++ plugin driver (spica_amd/host/plugin_demo)."""
+        fz = self.freeze()
+        with open(path, "wb") as f:
+            f.write(b"SPKS")
+            f.write(np.int32(1).tobytes())
+            f.write(np.int64(len(fz["verts"])).tobytes())
+            f.write(fz["verts"].tobytes())
+            f.write(fz["mat_ids"].tobytes())
+            f.write(fz["light_ids"].tobytes())
+            f.write(np.int32(len(self.materials)).tobytes())
+            for m in self.materials:
+                f.write(bytes(_material_struct(m)))
+            f.write(np.int32(len(fz["area_tri"])).tobytes())
+            f.write(fz["area_tri"].tobytes())
+            f.write(fz["area_radiance"].tobytes())
+            f.write(bytes(camera.desc()))
+            f.write(np.array([spp, max_depth, sampler], dtype=np.int32).tobytes())
+            f.write(np.uint32(seed).tobytes())
+
     def desc(self):
         """SpkSceneDesc pointing into this scene's arrays (keep the Scene alive while used)."""
         fz = self.freeze()
