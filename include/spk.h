@@ -111,6 +111,11 @@ int spk_render_image(spk_render* render, float* rgb);
  * in float64 against a reference image `ref_rgb` ([h][w][3] float64, host memory). */
 int spk_render_relmse(spk_render* render, const double* ref_rgb, double eps, double* out);
 
+/* The same over n film pixels (film indices `pixels`, host int32) against ref_rgb [n][3]:
+ * a reference rendered on a pixel subset. */
+int spk_render_relmse_pixels(spk_render* render, int64_t n, const int32_t* pixels, const double* ref_rgb, double eps,
+                             double* out);
+
 typedef struct spk_render_stats {
     int64_t samples_done;   /* samples per pixel accumulated */
     int64_t paths;          /* camera paths traced */
@@ -127,8 +132,68 @@ typedef struct spk_render_stats {
     int64_t count_nodes;    /* count_visits = 1: node fetches of the closest-hit stage */
     int64_t count_tris;     /* count_visits = 1: triangle tests of the closest-hit stage */
     int64_t count_rays;     /* count_visits = 1: closest-hit rays those counts cover */
+    int64_t guided_vertices;/* continuations sampled by the learned proposal (guided renders) */
+    int64_t train_passes;   /* sample passes that recorded training data (guided renders) */
 } spk_render_stats;
 int spk_render_stats_get(spk_render* render, spk_render_stats* out);
+
+/* ---- Learned path guiding ------------------------------------------------------------
+ * No counterpart in spica: its PathIntegrator samples every continuation from the BSDF
+ * (sources/integrators/path/path.cc:83-94).  A guide is a 224-64-64-64-64-40 ReLU network
+ * (bf16 weights, fp32 accumulation) over one-blob codes of position, normal and outgoing
+ * direction whose 40 outputs define an 8-lobe von Mises-Fisher mixture over incident
+ * directions.  Attached to a render it is trained online by Adam on the render's own
+ * radiance samples during the first `train_spp` samples per pixel, and continuations at
+ * diffuse and glossy vertices are drawn from guide_fraction * guide + (1 - guide_fraction)
+ * * BSDF with the mixture density in the weight, so the estimator stays unbiased.
+ *
+ * Network inputs x (7 floats per sample, in [0,1]): position scaled to the scene box (3),
+ * then for the normal and the outgoing direction ((v.z + 1)/2, atan2(v.y, v.x)/2pi + 1/2). */
+typedef struct spk_guide spk_guide;
+
+typedef struct spk_guide_opts {
+    uint32_t seed;           /* weight initialisation */
+    int32_t batch_size;      /* training samples per Adam step (default 2^18, multiple of 64) */
+    int32_t steps_per_pass;  /* Adam steps after each training round (default 4) */
+    int32_t train_spp;       /* samples per pixel of online training (default 64) */
+    float learning_rate;     /* default 1e-2 */
+    float beta1, beta2, epsilon;  /* default 0.9, 0.999, 1e-8 */
+    float guide_fraction;    /* probability of sampling the proposal at a vertex (default 0.5) */
+    int32_t train_pass_spp;  /* samples per pixel per training round (default 4); each round
+                              * keeps 1/train_pass_spp of its vertices as training records */
+} spk_guide_opts;
+
+int spk_guide_opts_default(spk_guide_opts* out);
+int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out);
+int spk_guide_destroy(spk_guide* guide);
+int64_t spk_guide_num_params(void);
+/* Flat parameters in order W1[64][224] b1 W2[64][64] b2 W3 b3 W4 b4 W5[40][64] b5 (fp32).
+ * Setting them resets the Adam moments and step count. */
+int spk_guide_params_get(spk_guide* guide, float* out);
+int spk_guide_params_set(spk_guide* guide, const float* in);
+/* Host arrays: x [n][7], dirs [n][3] unit vectors.  out_raw [n][40] and log_pdf [n] may be NULL. */
+int spk_guide_eval(spk_guide* guide, int64_t n, const float* x, const float* dirs, float* out_raw, float* log_pdf);
+/* Directions drawn from the mixture with uniforms u [n][3] (lobe, then the vMF about its
+ * mean); dirs [n][3] and their density pdf [n]. */
+int spk_guide_sample(spk_guide* guide, int64_t n, const float* x, const float* u, float* dirs, float* pdf);
+/* Gradient of the mean over n <= batch_size samples of -weight * log q(dir), and that mean. */
+int spk_guide_grad(spk_guide* guide, int64_t n, const float* x, const float* dirs, const float* weight, float* grad,
+                   double* loss);
+/* `steps` Adam steps on one batch of n <= batch_size samples. */
+int spk_guide_train(spk_guide* guide, int64_t n, const float* x, const float* dirs, const float* weight, int32_t steps);
+
+typedef struct spk_guide_stats {
+    int64_t steps;           /* Adam steps taken */
+    int64_t last_records;    /* training records in the last training pass */
+    double last_loss;        /* mean -weight * log q of the last step's batch */
+    double last_weight;      /* mean weight of the last step's batch */
+} spk_guide_stats;
+int spk_guide_stats_get(spk_guide* guide, spk_guide_stats* out);
+
+/* Attach a guide to a render (NULL detaches: plain BSDF sampling again).  The render
+ * trains it during its first train_spp samples per pixel after attaching.  One guide
+ * serves one render at a time. */
+int spk_render_set_guide(spk_render* render, spk_guide* guide);
 
 /* Raw 32-bit words of the counter-based sample streams for n (pixel, sample, dim)
  * keys (host arrays).  on_device = 0 evaluates on the host, 1 on the GPU with the

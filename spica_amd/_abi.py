@@ -127,6 +127,32 @@ class SpkRenderStats(ctypes.Structure):
         ("count_nodes", ctypes.c_int64),
         ("count_tris", ctypes.c_int64),
         ("count_rays", ctypes.c_int64),
+        ("guided_vertices", ctypes.c_int64),
+        ("train_passes", ctypes.c_int64),
+    ]
+
+
+class SpkGuideOpts(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint32),
+        ("batch_size", ctypes.c_int32),
+        ("steps_per_pass", ctypes.c_int32),
+        ("train_spp", ctypes.c_int32),
+        ("learning_rate", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("guide_fraction", ctypes.c_float),
+        ("train_pass_spp", ctypes.c_int32),
+    ]
+
+
+class SpkGuideStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("last_records", ctypes.c_int64),
+        ("last_loss", ctypes.c_double),
+        ("last_weight", ctypes.c_double),
     ]
 
 
@@ -148,9 +174,22 @@ SIGNATURES = {
     "spk_render_film": (ctypes.c_int, [ctypes.c_void_p, c_d_p, c_d_p, c_d_p]),
     "spk_render_image": (ctypes.c_int, [ctypes.c_void_p, c_f_p]),
     "spk_render_relmse": (ctypes.c_int, [ctypes.c_void_p, c_d_p, ctypes.c_double, c_d_p]),
+    "spk_render_relmse_pixels": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_i32_p, c_d_p, ctypes.c_double, c_d_p]),
     "spk_render_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkRenderStats)]),
     "spk_sampler_u32": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_int64, c_u32_p, c_u32_p, c_u32_p,
                                        c_u32_p, ctypes.c_int32]),
+    "spk_guide_opts_default": (ctypes.c_int, [ctypes.POINTER(SpkGuideOpts)]),
+    "spk_guide_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(SpkGuideOpts), ctypes.POINTER(ctypes.c_void_p)]),
+    "spk_guide_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_guide_num_params": (ctypes.c_int64, []),
+    "spk_guide_params_get": (ctypes.c_int, [ctypes.c_void_p, c_f_p]),
+    "spk_guide_params_set": (ctypes.c_int, [ctypes.c_void_p, c_f_p]),
+    "spk_guide_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, c_f_p]),
+    "spk_guide_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, c_f_p]),
+    "spk_guide_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, c_f_p, c_d_p]),
+    "spk_guide_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, ctypes.c_int32]),
+    "spk_guide_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkGuideStats)]),
+    "spk_render_set_guide": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

@@ -225,6 +225,7 @@ struct spk_scene {
     DevBuf trace_work, trace_spill;
     int trace_grid = 0;
     int spill_cap = 0;
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {1.0f, 1.0f, 1.0f};  // bounding box of the triangles
     spk_scene_stats stats;
 };
 
@@ -245,6 +246,12 @@ struct spk_render {
     int64_t extend_launches = 0;
     int64_t extend_rays_timed = 0;
     std::vector<hipEvent_t> events;
+    // learned proposal
+    spk_guide* guide = nullptr;
+    DevBuf qg[4], rec[5];
+    uint32_t rec_cap = 0;
+    int64_t guide_trained = 0;   // training samples per pixel done since attaching
+    int64_t train_passes = 0;
 };
 
 extern "C" {
@@ -522,6 +529,19 @@ int spk_scene_create(const spk_scene_desc* d, spk_scene** out) {
     sd.env = env;
 
     s->spill_cap = std::max(8, bvh.max_depth + 2);
+    for (int k = 0; k < 3; k++) {
+        s->lo[k] = 3.0e38f;
+        s->hi[k] = -3.0e38f;
+    }
+    for (int64_t i = 0; i < 9 * n; i++) {
+        s->lo[i % 3] = std::min(s->lo[i % 3], d->verts[i]);
+        s->hi[i % 3] = std::max(s->hi[i % 3], d->verts[i]);
+    }
+    if (n == 0)
+        for (int k = 0; k < 3; k++) {
+            s->lo[k] = 0.0f;
+            s->hi[k] = 1.0f;
+        }
     s->stats.num_tris = n;
     s->stats.num_nodes = (int64_t)nodes.size();
     s->stats.num_leaves = bvh.num_leaves;
@@ -689,6 +709,7 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     r->shape.block = o.block_size;
     r->shape.grid_trace = prop.multiProcessorCount * std::max(1, trace_blocks_per_cu(o.block_size));
     r->shape.grid_stream = prop.multiProcessorCount * 8;
+    r->shape.grid_guide = prop.multiProcessorCount;
     if ((rc = r->spill.alloc((size_t)r->shape.grid_trace * o.block_size * s->spill_cap * 4))) return rc;
     if ((rc = r->cfg_dev.alloc(sizeof(ConfigDev))) || (rc = r->ps_dev.alloc(sizeof(PassState)))) return rc;
     if ((rc = r->scratch.alloc(4096 * sizeof(double)))) return rc;
@@ -734,26 +755,40 @@ int spk_render_destroy(spk_render* r) {
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
     DevBuf* bufs[] = {&r->qa_o[0], &r->qa_o[1], &r->qa_d[0], &r->qa_d[1], &r->qa_b[0], &r->qa_b[1], &r->hit,
                       &r->qs_o, &r->qs_d, &r->qs_c, &r->qm_o, &r->qm_d, &r->qm_c, &r->path_L, &r->film_sum,
-                      &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch};
+                      &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch,
+                      &r->qg[0], &r->qg[1], &r->qg[2], &r->qg[3], &r->rec[0], &r->rec[1], &r->rec[2], &r->rec[3],
+                      &r->rec[4]};
     for (DevBuf* b : bufs) b->release();
     delete r;
     return SPK_OK;
 }
 
-int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
-    clear_error();
-    if (!r || n_spp < 0) {
-        set_error("spk_render_run: bad arguments");
-        return SPK_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+GuideNetDev guide_net(spk_render* r, bool record) {
+    GuideNetDev n;
+    memset(&n, 0, sizeof(n));
+    spk_guide* g = r->guide;
+    n.wf = g->wf.as<uint16_t>();
+    n.bias = g->bias.as<float>();
+    for (int k = 0; k < 3; k++) {
+        float ext = r->scene->hi[k] - r->scene->lo[k];
+        n.lo[k] = r->scene->lo[k];
+        n.inv_ext[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
     }
-    if (n_spp == 0) return SPK_OK;
-    SPK_CHECK_HIP(hipSetDevice(r->device));
+    n.frac = g->opts.guide_fraction;
+    n.record = record ? 1 : 0;
+    for (int k = 0; k < 5; k++) n.rec[k] = r->rec[k].as<float4>();
+    n.rec_cap = r->rec_cap;
+    n.rec_count = g->state.as<uint32_t>();  // GuideState::rec_count, its first word
+    return n;
+}
+
+// Samples [first, first + n_spp) as passes of the compiled pipeline.
+int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* net) {
     ConfigDev& c = r->cfg;
-    int64_t first = (int64_t)r->desc.first_sample + r->samples_done;
-    if (first + n_spp >= (int64_t(1) << 31)) {
-        set_error("spk_render_run: sample index overflow");
-        return SPK_ERR_INVALID;
-    }
     c.first_sample = (uint32_t)first;
     c.end_sample = (uint32_t)(first + n_spp);
     SPK_CHECK_HIP(hipMemcpyAsync(r->cfg_dev.p, &c, sizeof(ConfigDev), hipMemcpyHostToDevice, r->stream));
@@ -776,10 +811,10 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
     }
     for (int64_t p = 0; p < passes; p++) {
         hipEvent_t* ev = r->opts.profile ? &r->events[(size_t)p * nev] : nullptr;
-        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0, r->opts.count_visits != 0)))
+        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0, r->opts.count_visits != 0,
+                              net)))
             return rc;
     }
-    r->samples_done += n_spp;
     if (r->opts.profile) {
         SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
         PassState ps;
@@ -793,7 +828,93 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
                 r->extend_launches++;
             }
     }
+    return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
+    clear_error();
+    if (!r || n_spp < 0) {
+        set_error("spk_render_run: bad arguments");
+        return SPK_ERR_INVALID;
+    }
+    if (n_spp == 0) return SPK_OK;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    int64_t first = (int64_t)r->desc.first_sample + r->samples_done;
+    if (first + n_spp >= (int64_t(1) << 31)) {
+        set_error("spk_render_run: sample index overflow");
+        return SPK_ERR_INVALID;
+    }
+    int rc;
+    if (!r->guide) {
+        if ((rc = run_passes(r, first, n_spp, nullptr))) return rc;
+        r->samples_done += n_spp;
+    } else {
+        // Online training: rounds of train_pass_spp samples per pixel, each keeping a
+        // 1/train_pass_spp subsample of its vertices as records and followed by Adam steps
+        // on them; then the remaining samples with the network fixed.
+        spk_guide* g = r->guide;
+        int32_t left = n_spp;
+        while (left > 0) {
+            const bool train = r->guide_trained < g->opts.train_spp && g->opts.steps_per_pass > 0;
+            const int32_t k = train ? (int32_t)std::min<int64_t>({(int64_t)left, (int64_t)g->opts.train_pass_spp,
+                                                                  g->opts.train_spp - r->guide_trained})
+                                    : left;
+            // rec_count is the first word of the guide state
+            if (train && (rc = zero_u32(g->state.as<uint32_t>(), 1, r->stream))) return rc;
+            GuideNetDev net = guide_net(r, train);
+            net.rec_keep = 1.0f / (float)k;
+            if ((rc = run_passes(r, first, k, &net))) return rc;
+            if (train) {
+                const TrainBufs tb = g->train_bufs(g->bcap);
+                for (int st = 0; st < g->opts.steps_per_pass; st++) {
+                    uint32_t h0 = g->opts.seed, h1 = (uint32_t)g->steps, h2 = 0x6a09e667u, h3 = 0xbb67ae85u;
+                    pcg4d(h0, h1, h2, h3);
+                    if ((rc = launch_guide_step(g->bufs(), tb, &net, r->path_L.as<float4>(), g->adam(), h0, 0u, true,
+                                                nullptr, r->stream)))
+                        return rc;
+                    g->steps++;
+                }
+                r->guide_trained += k;
+                r->train_passes++;
+            }
+            first += k;
+            left -= k;
+            r->samples_done += k;
+        }
+    }
     if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    return SPK_OK;
+}
+
+int spk_render_set_guide(spk_render* r, spk_guide* g) {
+    clear_error();
+    if (!r) return SPK_ERR_INVALID;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    if (g && g->device != r->device) {
+        set_error("spk_render_set_guide: guide and render on different devices");
+        return SPK_ERR_INVALID;
+    }
+    if (g) SPK_CHECK_HIP(hipStreamSynchronize(g->stream));
+    r->guide = g;
+    r->guide_trained = 0;
+    if (!g || r->qg[0].p) return SPK_OK;
+    int rc;
+    const size_t Q = r->capacity;
+    for (int k = 0; k < 4; k++)
+        if ((rc = r->qg[k].alloc(Q * 16))) return rc;
+    // training records of one round: about one kept vertex per pixel, room for four
+    r->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)r->cfg.npix * 4u, uint64_t(1) << 24);
+    for (int k = 0; k < 5; k++)
+        if ((rc = r->rec[k].alloc((size_t)r->rec_cap * 16))) return rc;
+    r->bufs.qg[0] = r->qg[0].as<float4>();
+    r->bufs.qg[1] = r->qg[1].as<float4>();
+    r->bufs.qg[2] = r->qg[2].as<float4>();
+    r->bufs.qg[3] = r->qg[3].as<float4>();
     return SPK_OK;
 }
 
@@ -866,6 +987,30 @@ int spk_render_relmse(spk_render* r, const double* ref_rgb, double eps, double* 
     return SPK_OK;
 }
 
+int spk_render_relmse_pixels(spk_render* r, int64_t n, const int32_t* pixels, const double* ref_rgb, double eps,
+                             double* out) {
+    clear_error();
+    if (!r || n <= 0 || !pixels || !ref_rgb || !out) return SPK_ERR_INVALID;
+    for (int64_t i = 0; i < n; i++)
+        if (pixels[i] < 0 || (uint32_t)pixels[i] >= r->cfg.npix) {
+            set_error("spk_render_relmse_pixels: pixel %d outside the film", pixels[i]);
+            return SPK_ERR_INVALID;
+        }
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    DevBuf px, ref;
+    int rc;
+    if ((rc = px.upload(pixels, (size_t)n * 4)) || (rc = ref.upload(ref_rgb, (size_t)n * 3 * sizeof(double)))) return rc;
+    const int nblocks = 1024;
+    if ((rc = launch_relmse_pixels(r->bufs, n, px.as<int32_t>(), ref.as<double>(), eps, r->scratch.as<double>(), nblocks,
+                                   r->stream)))
+        return rc;
+    double total = 0.0;
+    SPK_CHECK_HIP(hipMemcpyAsync(&total, r->scratch.as<double>() + nblocks, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    *out = total / (double)(n * 3);
+    return SPK_OK;
+}
+
 int spk_render_stats_get(spk_render* r, spk_render_stats* out) {
     clear_error();
     if (!r || !out) return SPK_ERR_INVALID;
@@ -886,6 +1031,8 @@ int spk_render_stats_get(spk_render* r, spk_render_stats* out) {
     out->count_nodes = (int64_t)ps.count_nodes;
     out->count_tris = (int64_t)ps.count_tris;
     out->count_rays = (int64_t)ps.count_rays;
+    out->guided_vertices = (int64_t)ps.total_guided;
+    out->train_passes = r->train_passes;
     return SPK_OK;
 }
 

@@ -109,11 +109,13 @@ struct PassState {
     uint32_t workA[kMaxDepthCap + 2];
     uint32_t workS[kMaxDepthCap + 2];
     uint32_t workM[kMaxDepthCap + 2];
+    uint32_t countG[kMaxDepthCap + 2];  // vertices handed to the learned proposal
     unsigned long long total_paths;
     unsigned long long total_extend;
     unsigned long long total_shadow;
     unsigned long long total_mis;
     unsigned long long total_samples;
+    unsigned long long total_guided;
     unsigned long long count_nodes;  // extend traversal node fetches (counting passes)
     unsigned long long count_tris;   // extend traversal triangle tests (counting passes)
     unsigned long long count_rays;   // extend rays traced in counting passes

@@ -1,0 +1,320 @@
+// spk_guide.h -- learned path-guiding proposal, device side: the one-blob encoded
+// 224-64-64-64-64-40 ReLU network on gfx950 BF16 MFMA (v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation) and the 8-lobe von Mises-Fisher mixture it outputs.
+//
+// Register layout of one 16x16x32 MFMA (64 lanes, g = lane / 16, r = lane % 16):
+//   A (16 x 32):  lane holds A[r][8g .. 8g+7]
+//   B (32 x 16):  lane holds B[8g .. 8g+7][r]
+//   C (16 x 16):  lane holds C[4g + i][r], i = 0..3
+// Layers are computed transposed, C = W * H^T: weights are the A operand (rows =
+// output units), the queries are the 16 columns.  A layer's C tiles hold units
+// 16t + 4g + i of query r; packing two tiles (t = 2c, 2c+1) into the B operand of the
+// next layer gives K-slot 32c + 8g + j the unit 16(2c + j/4) + 4g + j%4.  The next
+// layer's weight columns are stored in that slot order (slot_unit()), so activations
+// never leave registers between layers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace spk {
+
+constexpr int kGIn = 7;                  // position (3), normal (2), outgoing direction (2)
+constexpr int kGBins = 32;               // one-blob bins per input
+constexpr int kGEnc = kGIn * kGBins;     // 224
+constexpr int kGHid = 64;
+constexpr int kGLobes = 8;
+constexpr int kGOut = 5 * kGLobes;       // per lobe: logit, mean (3), log concentration
+constexpr float kGLogKappaLo = -4.0f, kGLogKappaHi = 7.0f;
+
+// Flat parameter vector (logical order): W1[64][224] b1 W2[64][64] b2 W3 b3 W4 b4 W5[40][64] b5.
+constexpr int kGOffW1 = 0, kGOffB1 = 14336, kGOffW2 = 14400, kGOffB2 = 18496, kGOffW3 = 18560, kGOffB3 = 22656,
+              kGOffW4 = 22720, kGOffB4 = 26816, kGOffW5 = 26880, kGOffB5 = 29440, kGParams = 29480;
+// Forward bf16 image: W1 [64][224] natural, W2..W4 [64][64] and W5 [48][64] in slot order.
+constexpr int kGFwdW1 = 0, kGFwdW2 = 14336, kGFwdW3 = 18432, kGFwdW4 = 22528, kGFwdW5 = 26624, kGFwdSize = 29696;
+// Biases in fp32: b1..b4 at 64 l, b5 at 256 (48 with zero padding).
+constexpr int kGBiasSize = 304;
+// Backward bf16 image: WT2..WT4 [64][64] (rows = input unit, K in slot order of the
+// layer's outputs) and WT5 [64][64] (K = output index 0..39, zero padded).
+constexpr int kGBwdWT2 = 0, kGBwdWT3 = 4096, kGBwdWT4 = 8192, kGBwdWT5 = 12288, kGBwdSize = 16384;
+
+__host__ __device__ constexpr int slot_unit(int s) {
+    return 16 * (2 * (s >> 5) + ((s & 7) >> 2)) + 4 * ((s >> 3) & 3) + (s & 3);
+}
+__host__ __device__ constexpr int unit_slot(int u) {
+    return 32 * ((u >> 4) >> 1) + 8 * ((u >> 2) & 3) + 4 * ((u >> 4) & 1) + (u & 3);
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+// Round to nearest even; inputs here are finite.
+__host__ __device__ inline uint32_t bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
+__device__ __forceinline__ f32x4_t mfma_bf16(u32x4_t a, u32x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                   0, 0, 0);
+}
+__device__ __forceinline__ u32x4_t ld16(const uint16_t* p) { return *reinterpret_cast<const u32x4_t*>(p); }
+
+// Bins 8g .. 8g+7 of the one-blob code of x: exp(-(32x - b - 1/2)^2 / 2).
+__device__ __forceinline__ u32x4_t blob8(float x, int g) {
+    const float a = (float)kGBins * x - 8.0f * (float)g - 0.5f;
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float d = a - (float)j;
+        v[j] = __expf(-0.5f * d * d);
+    }
+    u32x4_t o;
+    o[0] = pack_bf16(v[0], v[1]);
+    o[1] = pack_bf16(v[2], v[3]);
+    o[2] = pack_bf16(v[4], v[5]);
+    o[3] = pack_bf16(v[6], v[7]);
+    return o;
+}
+
+// Bias + ReLU on the C tiles of a 64-unit layer, packed as the next layer's two B chunks.
+template <int NCT>
+__device__ __forceinline__ void relu_pack(const f32x4_t (&acc)[4][NCT], const float* bias, int g, u32x4_t (&h)[2][NCT]) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        float b0[4], b1[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            b0[i] = bias[16 * (2 * c) + 4 * g + i];
+            b1[i] = bias[16 * (2 * c + 1) + 4 * g + i];
+        }
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            const f32x4_t x = acc[2 * c][ct], y = acc[2 * c + 1][ct];
+            u32x4_t o;
+            o[0] = pack_bf16(fmaxf(x[0] + b0[0], 0.0f), fmaxf(x[1] + b0[1], 0.0f));
+            o[1] = pack_bf16(fmaxf(x[2] + b0[2], 0.0f), fmaxf(x[3] + b0[3], 0.0f));
+            o[2] = pack_bf16(fmaxf(y[0] + b1[0], 0.0f), fmaxf(y[1] + b1[1], 0.0f));
+            o[3] = pack_bf16(fmaxf(y[2] + b1[2], 0.0f), fmaxf(y[3] + b1[3], 0.0f));
+            h[c][ct] = o;
+        }
+    }
+}
+
+struct NoHook {
+    __device__ __forceinline__ void enc(int, int, const u32x4_t&) {}
+    template <int NCT>
+    __device__ __forceinline__ void act(int, const u32x4_t (&)[2][NCT]) {}
+};
+
+// Forward pass of NCT column tiles (16 queries each) for the calling wave.  feat(ct, d)
+// returns input d of query 16 ct + r for the calling lane.  On return out[mt][ct][i]
+// holds output unit 16 mt + 4g + i of query 16 ct + r (units >= 40 are padding).
+template <int NCT, class Feat, class Hook>
+__device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, const float* __restrict__ bias, Feat feat,
+                                            f32x4_t (&out)[3][NCT], Hook& hook) {
+    const int lane = __lane_id(), g = lane >> 4, r = lane & 15;
+    f32x4_t acc[4][NCT];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < kGIn; c++) {
+        u32x4_t b[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            b[ct] = blob8(feat(ct, c), g);
+            hook.enc(c, ct, b[ct]);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const u32x4_t a = ld16(w + kGFwdW1 + (16 * mt + r) * kGEnc + 32 * c + 8 * g);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = mfma_bf16(a, b[ct], acc[mt][ct]);
+        }
+    }
+    u32x4_t h[2][NCT];
+    relu_pack<NCT>(acc, bias, g, h);
+    hook.template act<NCT>(1, h);
+    const int offs[3] = {kGFwdW2, kGFwdW3, kGFwdW4};
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const u32x4_t a = ld16(w + offs[l] + (16 * mt + r) * kGHid + 32 * c + 8 * g);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = mfma_bf16(a, h[c][ct], acc[mt][ct]);
+            }
+        relu_pack<NCT>(acc, bias + 64 * (l + 1), g, h);
+        hook.template act<NCT>(l + 2, h);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) out[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++) {
+            const u32x4_t a = ld16(w + kGFwdW5 + (16 * mt + r) * kGHid + 32 * c + 8 * g);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) out[mt][ct] = mfma_bf16(a, h[c][ct], out[mt][ct]);
+        }
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float b = bias[256 + 16 * mt + 4 * g + i];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) out[mt][ct][i] += b;
+        }
+}
+
+// Hides a pointer's value from the optimiser for one loop iteration, so per-sample loops
+// reload weights instead of hoisting every operand into registers (which spills).
+template <class T>
+__device__ __forceinline__ T* opaque(T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// Orders LDS traffic between the lanes of one wave (no block barrier: waves of a
+// persistent block run different iteration counts).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- inputs ------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 dir_features(float x, float y, float z) {
+    return make_float2(0.5f * (z + 1.0f), atan2f(y, x) * 0.15915494309189535f + 0.5f);
+}
+
+// ---- the vMF mixture -------------------------------------------------------------------------------
+struct GuideMix {
+    float mx[kGLobes], my[kGLobes], mz[kGLobes];  // unit means
+    float kappa[kGLobes];
+    float logpi[kGLobes];
+    float lognorm[kGLobes];                        // log(kappa / (2 pi (1 - e^{-2 kappa})))
+};
+
+// o: the 40 raw outputs of one query (lobe k at 5k .. 5k+4).
+template <class Out>
+__device__ __forceinline__ void mix_from_outputs(Out o, GuideMix& m) {
+    float amax = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) amax = fmaxf(amax, o(5 * k));
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) s += __expf(o(5 * k) - amax);
+    const float ls = __logf(s) + amax;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) {
+        m.logpi[k] = o(5 * k) - ls;
+        float x = o(5 * k + 1), y = o(5 * k + 2), z = o(5 * k + 3);
+        float inv = rsqrtf(fmaxf(x * x + y * y + z * z, 1.0e-12f));
+        m.mx[k] = x * inv;
+        m.my[k] = y * inv;
+        m.mz[k] = z * inv;
+        float kap = __expf(fminf(fmaxf(o(5 * k + 4), kGLogKappaLo), kGLogKappaHi));
+        m.kappa[k] = kap;
+        m.lognorm[k] = __logf(kap) - 1.8378770664093453f /* log 2pi */ - __logf(-expm1f(-2.0f * kap));
+    }
+}
+
+__device__ __forceinline__ float mix_log_pdf(const GuideMix& m, float wx, float wy, float wz) {
+    float t[kGLobes], mx = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) {
+        float c = m.mx[k] * wx + m.my[k] * wy + m.mz[k] * wz;
+        t[k] = m.logpi[k] + m.lognorm[k] + m.kappa[k] * (c - 1.0f);
+        mx = fmaxf(mx, t[k]);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) s += __expf(t[k] - mx);
+    return mx + __logf(s);
+}
+
+// Basis around unit n (Duff et al., JCGT 2017).
+__device__ __forceinline__ void onb(float nx, float ny, float nz, float3& t1, float3& t2) {
+    const float s = nz >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (s + nz);
+    const float b = nx * ny * a;
+    t1 = make_float3(1.0f + s * nx * nx * a, s * b, -s * nx);
+    t2 = make_float3(b, s + ny * ny * a, -ny);
+}
+
+// u0 picks the lobe through the CDF of pi; (u1, u2) invert the vMF about its mean.
+__device__ __forceinline__ float3 mix_sample(const GuideMix& m, float u0, float u1, float u2) {
+    int k = kGLobes - 1;
+    float cdf = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kGLobes - 1; j++) {
+        cdf += __expf(m.logpi[j]);
+        if (u0 < cdf && k == kGLobes - 1) k = j;
+    }
+    // arithmetic select: an indexed read would move the lobe arrays to scratch memory
+    float mx = 0.0f, my = 0.0f, mz = 0.0f, kap = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kGLobes; j++) {
+        const float sel = j == k ? 1.0f : 0.0f;
+        mx += sel * m.mx[j];
+        my += sel * m.my[j];
+        mz += sel * m.mz[j];
+        kap += sel * m.kappa[j];
+    }
+    const float v = 1.0f - u1;
+    const float cz = fminf(fmaxf(1.0f + log1pf(v * expm1f(-2.0f * kap)) / kap, -1.0f), 1.0f);
+    const float sz = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
+    float sp, cp;
+    __sincosf(6.283185307179586f * u2, &sp, &cp);
+    float3 t1, t2;
+    onb(mx, my, mz, t1, t2);
+    return make_float3(cz * mx + sz * (cp * t1.x + sp * t2.x), cz * my + sz * (cp * t1.y + sp * t2.y),
+                       cz * mz + sz * (cp * t1.z + sp * t2.z));
+}
+
+// ---- device-side guide state shared by the render and training kernels -----------------------------
+struct GuideNetDev {
+    const uint16_t* wf;    // forward bf16 image
+    const float* bias;     // fp32 biases
+    float lo[3];           // scene box -> [0,1]^3
+    float inv_ext[3];
+    float frac;            // probability of sampling the guide at a vertex
+    int record;            // 1: store training records in this pass
+    float rec_keep;        // fraction of continuing vertices recorded
+    float4* rec[5];        // (x0..x3) (x4..x6, path) (w_i) (L before) (beta before)
+    uint32_t rec_cap;
+    uint32_t* rec_count;
+};
+
+struct GuideState {
+    uint32_t rec_count;    // records appended in the current training pass (may exceed capacity)
+    uint32_t n_rec;        // usable records
+    uint32_t b_eff;        // samples in the current batch
+    uint32_t active;       // the current step runs
+    uint32_t t;            // Adam steps taken
+    uint32_t perm_a;
+    uint32_t perm_off;
+    uint32_t pad;
+    float loss_sum;        // sum over the batch of -(F/p) log q
+    float weight_sum;
+    uint32_t pad2[2];
+};
+
+}  // namespace spk

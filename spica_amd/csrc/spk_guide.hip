@@ -1,0 +1,520 @@
+// spk_guide.hip -- training and evaluation kernels of the path-guiding network.
+//
+// One Adam step over a batch of B samples (x, w_i, F/p):
+//   k_guide_step_begin  batch size and record permutation for this step (1 thread)
+//   k_guide_batch       training records of the last pass -> batch, with F/p from the path radiance
+//   k_guide_fb          fused forward + backward-data per 16-sample tile: stores the one-blob code,
+//                       hidden activations and the back-propagated deltas (bf16, unit-major)
+//   k_guide_dw          weight gradients dW = delta * a^T as MFMA tiles over sample ranges
+//                       (one partial per range, no atomics)
+//   k_guide_adam        sums the partials, Adam update of the fp32 master weights, and
+//                       rewrites the bf16 forward/backward images.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "spk_guide.h"
+#include "spk_internal.h"
+#include "spk_sampler.h"
+
+namespace spk {
+
+// ---- parameter images ------------------------------------------------------------------------------
+__device__ void write_param_images(int p, float th, uint16_t* wf, uint16_t* wb, float* bias) {
+    const uint16_t h = (uint16_t)bf16_bits(th);
+    if (p < kGOffB1) {
+        wf[kGFwdW1 + p] = h;
+        return;
+    }
+    if (p < kGOffW2) {
+        bias[p - kGOffB1] = th;
+        return;
+    }
+    if (p >= kGOffB5) {
+        bias[256 + (p - kGOffB5)] = th;
+        return;
+    }
+    if (p >= kGOffW5) {
+        const int q = p - kGOffW5, n = q >> 6, k = q & 63;
+        wf[kGFwdW5 + n * 64 + unit_slot(k)] = h;
+        wb[kGBwdWT5 + k * 64 + n] = h;
+        return;
+    }
+    // layers 2..4: blocks of 4096 weights + 64 biases
+    const int q = p - kGOffW2, l = q / 4160, o = q - l * 4160;
+    if (o >= 4096) {
+        bias[64 * (l + 1) + (o - 4096)] = th;
+        return;
+    }
+    const int n = o >> 6, k = o & 63;
+    wf[kGFwdW2 + 4096 * l + n * 64 + unit_slot(k)] = h;
+    wb[kGBwdWT2 + 4096 * l + k * 64 + unit_slot(n)] = h;
+}
+
+__global__ void k_guide_images(const float* __restrict__ params, uint16_t* wf, uint16_t* wb, float* bias) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kGParams) write_param_images(p, params[p], wf, wb, bias);
+}
+
+// ---- evaluation (C-ABI checks) ----------------------------------------------------------------------
+// Block of 256 = 4 waves, each taking 64 queries per iteration with the weights in LDS,
+// exactly as the render kernel does.
+template <int MODE>  // 0: outputs + log pdf of given directions; 1: sample directions
+__global__ __launch_bounds__(256) void k_guide_eval(const uint16_t* __restrict__ wf, const float* __restrict__ bias,
+                                                    int64_t n, const float4* __restrict__ x, const float4* __restrict__ aux,
+                                                    float* __restrict__ out_raw, float4* __restrict__ res) {
+    __shared__ __attribute__((aligned(16))) uint16_t w_lds[kGFwdSize];
+    __shared__ float b_lds[kGBiasSize];
+    __shared__ float obuf[4][64 * 41];
+    for (int i = threadIdx.x; i < kGFwdSize / 8; i += blockDim.x)
+        reinterpret_cast<u32x4_t*>(w_lds)[i] = reinterpret_cast<const u32x4_t*>(wf)[i];
+    for (int i = threadIdx.x; i < kGBiasSize; i += blockDim.x) b_lds[i] = bias[i];
+    __syncthreads();
+    const int lane = __lane_id(), wv = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+    float* ob = obuf[wv];
+    const int64_t waves = (int64_t)gridDim.x * 4;
+    for (int64_t base = ((int64_t)blockIdx.x * 4 + wv) * 64; base < n; base += waves * 64) {
+        const int64_t i = base + lane;
+        const bool active = i < n;
+        float4 x0 = make_float4(0.5f, 0.5f, 0.5f, 0.5f), x1 = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
+        if (active) {
+            x0 = x[2 * i];
+            x1 = x[2 * i + 1];
+        }
+        ob[lane * 8 + 0] = x0.x;
+        ob[lane * 8 + 1] = x0.y;
+        ob[lane * 8 + 2] = x0.z;
+        ob[lane * 8 + 3] = x0.w;
+        ob[lane * 8 + 4] = x1.x;
+        ob[lane * 8 + 5] = x1.y;
+        ob[lane * 8 + 6] = x1.z;
+        wave_lds_sync();
+        f32x4_t out[3][4];
+        NoHook hook;
+        mlp_forward<4>(opaque(w_lds), opaque(b_lds), [&](int ct, int d) { return ob[(16 * ct + r) * 8 + d]; }, out, hook);
+        wave_lds_sync();
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int u = 16 * mt + 4 * g + k;
+                    if (u < kGOut) ob[(16 * ct + r) * 41 + u] = out[mt][ct][k];
+                }
+        wave_lds_sync();
+        if (active) {
+            const float* o = ob + lane * 41;
+            if (out_raw)
+                for (int u = 0; u < kGOut; u++) out_raw[i * kGOut + u] = o[u];
+            GuideMix mix;
+            mix_from_outputs([&](int u) { return o[u]; }, mix);
+            const float4 a = aux[i];
+            if (MODE == 0) {
+                res[i] = make_float4(mix_log_pdf(mix, a.x, a.y, a.z), 0.0f, 0.0f, 0.0f);
+            } else {
+                float3 d = mix_sample(mix, a.x, a.y, a.z);
+                res[i] = make_float4(d.x, d.y, d.z, __expf(mix_log_pdf(mix, d.x, d.y, d.z)));
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+// ---- training -----------------------------------------------------------------------------------------
+__global__ void k_guide_step_begin(GuideState* gs, uint32_t batch, uint32_t rec_cap, uint32_t seed, uint32_t direct_n,
+                                   uint32_t bump_t) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t n = direct_n ? direct_n : min(gs->rec_count, rec_cap);
+    gs->n_rec = n;
+    gs->active = n > 0 ? 1u : 0u;
+    gs->b_eff = min(batch, n);
+    if (n > 0 && bump_t) gs->t += 1;
+    const uint32_t primes[4] = {1000003u, 1000033u, 1000037u, 1000039u};
+    uint32_t a = 1;
+    for (int k = 0; k < 4; k++)
+        if (n > 0 && n % primes[k] != 0) {
+            a = primes[k];
+            break;
+        }
+    uint32_t h0 = seed, h1 = 0x85ebca6bu, h2 = 0xc2b2ae35u, h3 = 0x27d4eb2fu;
+    pcg4d(h0, h1, h2, h3);
+    gs->perm_a = a;
+    gs->perm_off = n > 0 ? h0 % n : 0u;
+    gs->loss_sum = 0.0f;
+    gs->weight_sum = 0.0f;
+}
+
+__device__ __forceinline__ float safe_ratio(float a, float b) { return b > 0.0f ? a / b : 0.0f; }
+
+__global__ void k_guide_batch(GuideNetDev net, const float4* __restrict__ path_L, const GuideState* __restrict__ gs,
+                              TrainBufs tb) {
+    if (!gs->active) return;
+    const uint32_t b_eff = gs->b_eff, n = gs->n_rec, a = gs->perm_a, off = gs->perm_off;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < tb.ncols; j += gridDim.x * blockDim.x) {
+        float4 x0 = make_float4(0.5f, 0.5f, 0.5f, 0.5f), x1 = make_float4(0.5f, 0.5f, 0.5f, 0.0f),
+               d = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        if (j < b_eff) {
+            const uint32_t i = (uint32_t)(((uint64_t)j * a + off) % n);
+            const float4 r0 = net.rec[0][i], r1 = net.rec[1][i], r2 = net.rec[2][i], r3 = net.rec[3][i],
+                         r4 = net.rec[4][i];
+            const float4 L = path_L[__float_as_uint(r1.w)];
+            // F/p of the sampled direction: the radiance the path gathered after this vertex,
+            // divided by the throughput that reached it.
+            float w = 0.2126f * safe_ratio(L.x - r3.x, r4.x) + 0.7152f * safe_ratio(L.y - r3.y, r4.y) +
+                      0.0722f * safe_ratio(L.z - r3.z, r4.z);
+            if (!(w > 0.0f) || !(w < 1.0e30f)) w = 0.0f;
+            x0 = r0;
+            x1 = make_float4(r1.x, r1.y, r1.z, w);
+            d = r2;
+        }
+        tb.tb[0][j] = x0;
+        tb.tb[1][j] = x1;
+        tb.tb[2][j] = d;
+    }
+}
+
+// Query-major rows: a 64-unit activation or delta is stored as 64 bf16 per sample in slot
+// order (slot s holds unit slot_unit(s)), so each lane writes its two packed chunks as
+// two 16-byte vectors.
+__device__ __forceinline__ void store_row(uint16_t* row, int g, const u32x4_t& c0, const u32x4_t& c1) {
+    *reinterpret_cast<u32x4_t*>(row + 8 * g) = c0;
+    *reinterpret_cast<u32x4_t*>(row + 32 + 8 * g) = c1;
+}
+
+struct FbHook {
+    uint16_t* erow;     // one-blob code of this sample (224 bf16)
+    uint16_t* arow[4];  // hidden activations of this sample (64 bf16 each)
+    int g;
+    u32x4_t hs[4][2];
+    __device__ __forceinline__ void enc(int c, int, const u32x4_t& b) {
+        *reinterpret_cast<u32x4_t*>(erow + 32 * c + 8 * g) = b;
+    }
+    template <int NCT>
+    __device__ __forceinline__ void act_impl(int l, const u32x4_t (&h)[2][NCT]) {
+        hs[l - 1][0] = h[0][0];
+        hs[l - 1][1] = h[1][0];
+        store_row(arow[l - 1], g, h[0][0], h[1][0]);
+    }
+    template <int NCT>
+    __device__ __forceinline__ void act(int l, const u32x4_t (&h)[2][NCT]) {
+#pragma unroll
+        for (int k = 1; k <= 4; k++)
+            if (l == k) act_impl<NCT>(k, h);
+    }
+};
+
+// delta (C tiles, units 16t + 4g + i) masked by the forward activation (packed chunks) and
+// packed as B chunks for the next backward product.
+__device__ __forceinline__ void mask_pack(const f32x4_t (&d)[4], const u32x4_t (&h)[2], u32x4_t (&o)[2]) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t word = h[c][j >> 1];
+            const uint32_t bits = (j & 1) ? (word >> 16) : (word & 0xffffu);
+            const float dv = j < 4 ? d[2 * c][j] : d[2 * c + 1][j - 4];
+            v[j] = bits != 0u ? dv : 0.0f;
+        }
+        o[c][0] = pack_bf16(v[0], v[1]);
+        o[c][1] = pack_bf16(v[2], v[3]);
+        o[c][2] = pack_bf16(v[4], v[5]);
+        o[c][3] = pack_bf16(v[6], v[7]);
+    }
+}
+
+// Persistent blocks (one per CU): the forward and backward weight images are staged in
+// LDS once, then each wave takes 16-sample tiles.
+__global__ __launch_bounds__(256) void k_guide_fb(const uint16_t* __restrict__ wf, const float* __restrict__ bias,
+                                                  const uint16_t* __restrict__ wb, TrainBufs tb, GuideState* gs) {
+    __shared__ __attribute__((aligned(16))) uint16_t wf_lds[kGFwdSize];
+    __shared__ __attribute__((aligned(16))) uint16_t wb_lds[kGBwdSize];
+    __shared__ float b_lds[kGBiasSize];
+    __shared__ float dl[4][16 * 41];
+    if (!gs->active) return;
+    for (int i = threadIdx.x; i < kGFwdSize / 8; i += blockDim.x)
+        reinterpret_cast<u32x4_t*>(wf_lds)[i] = reinterpret_cast<const u32x4_t*>(wf)[i];
+    for (int i = threadIdx.x; i < kGBwdSize / 8; i += blockDim.x)
+        reinterpret_cast<u32x4_t*>(wb_lds)[i] = reinterpret_cast<const u32x4_t*>(wb)[i];
+    for (int i = threadIdx.x; i < kGBiasSize; i += blockDim.x) b_lds[i] = bias[i];
+    __syncthreads();
+    const uint32_t b_eff = gs->b_eff;
+    const int lane = __lane_id(), wv = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+    float* dlw = dl[wv];
+    const uint32_t ld = tb.ld;
+    float loss_acc = 0.0f, w_acc = 0.0f;
+    const uint32_t waves = gridDim.x * 4;
+    for (uint32_t tile = blockIdx.x * 4 + wv; tile * 16 < tb.ncols; tile += waves) {
+        const uint32_t q = tile * 16 + r;
+        const float4 x0 = tb.tb[0][q], x1 = tb.tb[1][q];
+        const float xin[7] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z};
+        FbHook hook;
+        hook.erow = tb.E + (size_t)q * kGEnc;
+#pragma unroll
+        for (int l = 0; l < 4; l++) hook.arow[l] = tb.act + ((size_t)l * ld + q) * 64;
+        hook.g = g;
+        f32x4_t out[3][1];
+        mlp_forward<1>(opaque(wf_lds), opaque(b_lds), [&](int, int d) { return xin[d]; }, out, hook);
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int u = 16 * mt + 4 * g + k;
+                if (u < kGOut) dlw[r * 41 + u] = out[mt][0][k];
+            }
+        wave_lds_sync();
+        if (g == 0) {
+            float o[kGOut];
+#pragma unroll
+            for (int u = 0; u < kGOut; u++) o[u] = dlw[r * 41 + u];
+            GuideMix m;
+            mix_from_outputs([&](int u) { return o[u]; }, m);
+            const float4 dir = tb.tb[2][q];
+            const float w = q < b_eff ? x1.w : 0.0f;
+            // responsibilities and -w * d log q / d outputs
+            float t[kGLobes], cs[kGLobes], mxv = -3.0e38f;
+#pragma unroll
+            for (int k = 0; k < kGLobes; k++) {
+                cs[k] = m.mx[k] * dir.x + m.my[k] * dir.y + m.mz[k] * dir.z;
+                t[k] = m.logpi[k] + m.lognorm[k] + m.kappa[k] * (cs[k] - 1.0f);
+                mxv = fmaxf(mxv, t[k]);
+            }
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kGLobes; k++) s += __expf(t[k] - mxv);
+            const float lq = mxv + __logf(s);
+            if (w > 0.0f) {
+                loss_acc += -w * lq;
+                w_acc += w;
+            }
+#pragma unroll
+            for (int k = 0; k < kGLobes; k++) {
+                const float gam = __expf(t[k] - lq), pi = __expf(m.logpi[k]), kap = m.kappa[k];
+                dlw[r * 41 + 5 * k] = -w * (gam - pi);
+                const float mlen = sqrtf(fmaxf(o[5 * k + 1] * o[5 * k + 1] + o[5 * k + 2] * o[5 * k + 2] +
+                                                   o[5 * k + 3] * o[5 * k + 3], 1.0e-12f));
+                const float sc = -w * gam * kap / mlen;
+                dlw[r * 41 + 5 * k + 1] = sc * (dir.x - cs[k] * m.mx[k]);
+                dlw[r * 41 + 5 * k + 2] = sc * (dir.y - cs[k] * m.my[k]);
+                dlw[r * 41 + 5 * k + 3] = sc * (dir.z - cs[k] * m.mz[k]);
+                const float rr = o[5 * k + 4];
+                const bool inside = rr > kGLogKappaLo && rr < kGLogKappaHi;
+                const float tk = 2.0f * kap;
+                const float dr = 1.0f - tk / expm1f(tk) + kap * (cs[k] - 1.0f);
+                dlw[r * 41 + 5 * k + 4] = inside ? -w * gam * dr : 0.0f;
+            }
+        }
+        wave_lds_sync();
+        // output deltas (natural order, zero beyond 40): global copy for dW5 and the B
+        // operand of the first backward product
+        u32x4_t bd[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int u = 32 * c + 8 * g + j;
+                v[j] = u < kGOut ? dlw[r * 41 + u] : 0.0f;
+            }
+            bd[c][0] = pack_bf16(v[0], v[1]);
+            bd[c][1] = pack_bf16(v[2], v[3]);
+            bd[c][2] = pack_bf16(v[4], v[5]);
+            bd[c][3] = pack_bf16(v[6], v[7]);
+        }
+        store_row(tb.dz + ((size_t)4 * ld + q) * 64, g, bd[0], bd[1]);
+        wave_lds_sync();
+        // backward through layers 5 .. 2
+        const int wt_off[4] = {kGBwdWT2, kGBwdWT3, kGBwdWT4, kGBwdWT5};
+        const uint16_t* wbt = opaque(wb_lds);
+#pragma unroll
+        for (int l = 5; l >= 2; l--) {
+            f32x4_t acc[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    const u32x4_t a = ld16(wbt + wt_off[l - 2] + (16 * mt + r) * 64 + 32 * c + 8 * g);
+                    acc[mt] = mfma_bf16(a, bd[c], acc[mt]);
+                }
+            u32x4_t nd[2];
+            mask_pack(acc, hook.hs[l - 2], nd);
+            store_row(tb.dz + ((size_t)(l - 2) * ld + q) * 64, g, nd[0], nd[1]);
+            bd[0] = nd[0];
+            bd[1] = nd[1];
+        }
+    }
+    // one atomic per wave for the loss monitor
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        loss_acc += __shfl_xor(loss_acc, o);
+        w_acc += __shfl_xor(w_acc, o);
+    }
+    if (lane == 0) {
+        atomicAdd(&gs->loss_sum, loss_acc);
+        atomicAdd(&gs->weight_sum, w_acc);
+    }
+}
+
+// Weight gradients.  blockIdx.x = tile: 0..3 layer 1 (64 code columns each), 4..6 layers
+// 2..4, 7 layer 5; blockIdx.y = sample range.  One wave accumulates up to 64 x 64 outputs
+// over its range, 32 samples at a time: the query-major rows are staged into LDS
+// transposed (unit-major), so each MFMA operand is one 16-byte LDS read of 8 samples.
+constexpr int kDwStride = 40;  // LDS row stride in bf16 (32 samples + pad, keeps 16 B alignment)
+
+__global__ __launch_bounds__(64) void k_guide_dw(TrainBufs tb, const GuideState* __restrict__ gs, float* partial,
+                                                 uint32_t cols_per_split) {
+    __shared__ __attribute__((aligned(16))) uint16_t sa[64 * kDwStride];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[64 * kDwStride];
+    if (!gs->active) return;
+    const int tile = blockIdx.x, lane = __lane_id(), g = lane >> 4, r = lane & 15;
+    int layer, kbase = 0;
+    if (tile < 4) {
+        layer = 1;
+        kbase = 64 * tile;
+    } else {
+        layer = tile - 2;
+    }
+    const int K = layer == 1 ? kGEnc : kGHid;       // columns of W (row length of the B rows)
+    const int nvalid = layer == 5 ? kGOut : kGHid;
+    const int nmt = layer == 5 ? 3 : 4;
+    const int ncols = min(64, K - kbase);             // 64 or 32
+    const int nkt = ncols / 16;
+    const size_t ld = tb.ld;
+    const uint16_t* A = tb.dz + (size_t)(layer - 1) * ld * 64;                       // [q][64]
+    const uint16_t* B = layer == 1 ? tb.E + kbase : tb.act + (size_t)(layer - 2) * ld * 64;  // [q][K] from kbase
+    const uint32_t q_lo = blockIdx.y * cols_per_split, q_hi = min(tb.ncols, q_lo + cols_per_split);
+    f32x4_t acc[4][4];
+    float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) acc[mt][kt] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t q0 = q_lo; q0 < q_hi; q0 += 32) {
+        // stage: 32 samples x 64 slots of A and 32 x ncols of B, written transposed
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            const int e = it * 64 + lane;        // 256 vectors of 8 bf16: sample e / 8, chunk e % 8
+            const int qs = e >> 3, ch = e & 7;
+            const u32x4_t va = *reinterpret_cast<const u32x4_t*>(A + (size_t)(q0 + qs) * 64 + 8 * ch);
+            u32x4_t vb = u32x4_t{0u, 0u, 0u, 0u};
+            if (8 * ch < ncols) vb = *reinterpret_cast<const u32x4_t*>(B + (size_t)(q0 + qs) * K + 8 * ch);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t wa = va[j >> 1], wbv = vb[j >> 1];
+                sa[(8 * ch + j) * kDwStride + qs] = (uint16_t)((j & 1) ? (wa >> 16) : (wa & 0xffffu));
+                sb[(8 * ch + j) * kDwStride + qs] = (uint16_t)((j & 1) ? (wbv >> 16) : (wbv & 0xffffu));
+            }
+        }
+        wave_lds_sync();
+        u32x4_t a[4], b[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) a[mt] = *reinterpret_cast<const u32x4_t*>(sa + (16 * mt + r) * kDwStride + 8 * g);
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) b[kt] = *reinterpret_cast<const u32x4_t*>(sb + (16 * kt + r) * kDwStride + 8 * g);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+                if (mt < nmt && kt < nkt) acc[mt][kt] = mfma_bf16(a[mt], b[kt], acc[mt][kt]);
+        if (kbase == 0) {
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) bsum[mt] += bf16_lo(a[mt][j]) + bf16_hi(a[mt][j]);
+        }
+        wave_lds_sync();
+    }
+    // Rows of A are slots of the layer's outputs (natural order for layer 5); columns of B
+    // are code bins (layer 1) or slots of the previous layer's outputs.
+    const int offW[5] = {kGOffW1, kGOffW2, kGOffW3, kGOffW4, kGOffW5};
+    const int offB[5] = {kGOffB1, kGOffB2, kGOffB3, kGOffB4, kGOffB5};
+    float* out = partial + (size_t)blockIdx.y * kGParams;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int sn = 16 * mt + 4 * g + i, sk = 16 * kt + r;
+                const int n = layer == 5 ? sn : slot_unit(sn);
+                const int k = layer == 1 ? kbase + sk : slot_unit(sk);
+                if (mt < nmt && kt < nkt && n < nvalid) out[offW[layer - 1] + n * K + k] = acc[mt][kt][i];
+            }
+    if (kbase == 0) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            float s = bsum[mt];
+            s += __shfl_xor(s, 16);
+            s += __shfl_xor(s, 32);
+            const int sn = 16 * mt + r;
+            const int n = layer == 5 ? sn : slot_unit(sn);
+            if (g == 0 && mt < nmt && n < nvalid) out[offB[layer - 1] + n] = s;
+        }
+    }
+}
+
+__global__ void k_guide_adam(float* params, float* m, float* v, const float* __restrict__ partial, int splits,
+                             const GuideState* __restrict__ gs, AdamCfg cfg, float* grad_out, uint16_t* wf, uint16_t* wb,
+                             float* bias) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kGParams || !gs->active) return;
+    float s = 0.0f;
+    for (int k = 0; k < splits; k++) s += partial[(size_t)k * kGParams + p];
+    const float gr = s / (float)gs->b_eff;
+    if (grad_out) {
+        grad_out[p] = gr;
+        return;
+    }
+    const float t = (float)gs->t;
+    const float mm = cfg.beta1 * m[p] + (1.0f - cfg.beta1) * gr;
+    const float vv = cfg.beta2 * v[p] + (1.0f - cfg.beta2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float mh = mm / (1.0f - powf(cfg.beta1, t));
+    const float vh = vv / (1.0f - powf(cfg.beta2, t));
+    const float th = params[p] - cfg.lr * mh / (sqrtf(vh) + cfg.eps);
+    params[p] = th;
+    write_param_images(p, th, wf, wb, bias);
+}
+
+// ---- launchers -----------------------------------------------------------------------------------------
+int launch_guide_images(const float* params, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_images, dim3((kGParams + 255) / 256), dim3(256), 0, st, params, wf, wb, bias);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n, const float4* x, const float4* aux,
+                      float* out_raw, float4* res, int grid, hipStream_t st) {
+    if (mode == 0)
+        hipLaunchKernelGGL(k_guide_eval<0>, dim3(grid), dim3(256), 0, st, wf, bias, n, x, aux, out_raw, res);
+    else
+        hipLaunchKernelGGL(k_guide_eval<1>, dim3(grid), dim3(256), 0, st, wf, bias, n, x, aux, out_raw, res);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net, const float4* path_L,
+                      const AdamCfg& cfg, uint32_t seed, uint32_t direct_n, bool bump_t, float* grad_out,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_step_begin, dim3(1), dim3(64), 0, st, gb.gs, tb.batch, net ? net->rec_cap : 0u, seed,
+                       direct_n, bump_t ? 1u : 0u);
+    if (!direct_n) hipLaunchKernelGGL(k_guide_batch, dim3(1024), dim3(256), 0, st, *net, path_L, (const GuideState*)gb.gs, tb);
+    const uint32_t fb_blocks = std::max(1u, std::min((uint32_t)gb.grid, (tb.ncols / 16 + 3) / 4));
+    hipLaunchKernelGGL(k_guide_fb, dim3(fb_blocks), dim3(256), 0, st, (const uint16_t*)gb.wf, (const float*)gb.bias,
+                       (const uint16_t*)gb.wb, tb, gb.gs);
+    const uint32_t cps = kGuideColsPerSplit;
+    const uint32_t splits = (tb.ncols + cps - 1) / cps;
+    hipLaunchKernelGGL(k_guide_dw, dim3(8, splits), dim3(64), 0, st, tb, (const GuideState*)gb.gs, gb.partial, cps);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gb.params, gb.m, gb.v,
+                       (const float*)gb.partial, (int)splits, (const GuideState*)gb.gs, cfg, grad_out, gb.wf, gb.wb,
+                       gb.bias);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+}  // namespace spk

@@ -9,6 +9,7 @@
 
 #include "../../include/spk.h"
 #include "spk_device.h"
+#include "spk_guide.h"
 
 namespace spk {
 
@@ -87,6 +88,7 @@ struct RenderBufs {
     float4* qm_o;
     float4* qm_d;
     float4* qm_c;
+    float4* qg[4];      // guide queue: position|path, wo|flags, beta|pixel, shading normal|triangle
     float4* path_L;     // per path radiance
     float4* film_sum;   // rgb + count
     float4* film_c;     // Kahan compensation
@@ -101,14 +103,65 @@ struct RenderBufs {
 struct LaunchShape {
     int grid_trace;     // persistent traversal blocks
     int grid_stream;    // grid-stride blocks for shading / streaming kernels
+    int grid_guide;     // persistent blocks of the guide kernel (one per CU)
     int block;          // threads per block
 };
 
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls,
-                hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count);
+                hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide);
 int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st);
 int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st);
 int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial,
                   int nblocks, hipStream_t st);
+int launch_relmse_pixels(const RenderBufs& b, int64_t n, const int32_t* pixels, const double* ref, double eps,
+                         double* out_partial, int nblocks, hipStream_t st);
+
+// ---- path guiding (spk_guide.hip) -------------------------------------------------
+struct TrainBufs {
+    float4* tb[3];      // batch: (x0..x3), (x4..x6, F/p), (direction)
+    uint16_t* E;        // one-blob code [ld][224]
+    uint16_t* act;      // hidden activations [4][ld][64], slot order
+    uint16_t* dz;       // deltas: layers 1..4 [4][ld][64] slot order, layer 5 [ld][64] natural order
+    uint32_t ld;        // samples per array (batch capacity)
+    uint32_t ncols;     // columns processed, multiple of 64
+    uint32_t batch;     // samples per step
+};
+struct AdamCfg {
+    float lr, beta1, beta2, eps;
+};
+struct GuideBufsDev {
+    float* params;
+    float* m;
+    float* v;
+    uint16_t* wf;
+    uint16_t* wb;
+    float* bias;
+    float* partial;     // [splits][kGParams]
+    GuideState* gs;
+    int grid;           // persistent blocks for the training kernel (compute units)
+};
+constexpr uint32_t kGuideColsPerSplit = 1024;
+
+int launch_guide_images(const float* params, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t st);
+int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n, const float4* x, const float4* aux,
+                      float* out_raw, float4* res, int grid, hipStream_t st);
+int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net, const float4* path_L,
+                      const AdamCfg& cfg, uint32_t seed, uint32_t direct_n, bool bump_t, float* grad_out,
+                      hipStream_t st);
 
 }  // namespace spk
+
+// The guide handle (C ABI type) shared by the render and guide translation units.
+struct spk_guide {
+    int device = 0;
+    hipStream_t stream = nullptr;       // for the stand-alone entry points
+    spk_guide_opts opts;
+    spk::DevBuf params, m, v, wf, wb, bias, partial, state, grad;
+    spk::DevBuf tb[3], E, act, dz;
+    uint32_t bcap = 0;                  // batch capacity (samples)
+    int cus = 256;                      // compute units of the device
+    int64_t steps = 0;                  // Adam steps launched
+    spk::GuideBufsDev bufs() const;
+    spk::TrainBufs train_bufs(uint32_t ncols) const;
+    spk::AdamCfg adam() const;
+};

@@ -18,6 +18,7 @@
 // * film: per pixel, the samples of the pass are added in sample order into
 //   Kahan-compensated float sums of L and L^2: deterministic and reproducible.
 #include "spk_internal.h"
+#include "spk_guide.h"
 #include "spk_sampler.h"
 
 namespace spk {
@@ -337,6 +338,7 @@ __global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps) {
         ps->workA[i] = 0;
         ps->workS[i] = 0;
         ps->workM[i] = 0;
+        ps->countG[i] = 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) ps->countA[0] = n_sh;
@@ -852,8 +854,43 @@ __device__ float area_pdf(const LightDev& L, float3 p, float3 ng, float3 wi) {
 
 __device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
 
+// Material lobe and parameters of a surface hit (bsdf.cc:14-21).
+__device__ __forceinline__ void bsdf_material(const SceneDev& sc, uint32_t mbits, Bsdf& b) {
+    const MatDev M = sc.mats[mbits & 0x7fffffffu];
+    b.lobe = __float_as_int(M.a.w);
+    b.type = lobe_type(b.lobe);
+    b.R = xyz(M.a);
+    b.T = xyz(M.b);
+    b.etaB = M.b.w;
+    b.eta = xyz(M.c);
+    b.alpha = M.c.w;
+    b.k = xyz(M.d);
+}
+
+// Shading frame: the geometric frame, or the interpolated normal's frame when the
+// triangle is smooth and the normals differ (triangle.cc:127-155).
+__device__ __forceinline__ void bsdf_frame(float3 ng, bool smooth, float3 ns, Bsdf& b) {
+    float3 dpdu, dpdv;
+    coordinate_system(ng, dpdu, dpdv);
+    b.n = ng;
+    b.t = dpdu;
+    b.b = dpdv;
+    if (smooth && fabsf(dot(ns, ng)) < 1.0f - 1.0e-6f) {
+        float3 ss = normalize(cross(ns, ng));
+        float3 ts = normalize(cross(ns, ss));
+        b.n = normalize(cross(ss, ts));
+        b.t = ss;
+        b.b = ts;
+    }
+}
+
+// Vertices whose continuation the learned proposal samples: the non-specular lobes.
+__device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
+    return (b.lobe == LOBE_LAMBERT && !black(b.R)) || b.lobe == LOBE_MICROREFL;
+}
+
 // ---- shade ---------------------------------------------------------------------------------------
-template <int BLOCK>
+template <int BLOCK, bool GUIDED>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                  int depth) {
     PassState* ps = rb.ps;
@@ -869,7 +906,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < count;
-        bool pushA = false, pushS = false, pushM = false;
+        bool pushA = false, pushS = false, pushM = false, pushG = false;
+        float4 gq0 = make_float4(0, 0, 0, 0), gq1 = gq0, gq3 = gq0;
         float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0);
         uint32_t nA_flags = 0;
         float3 s_o = mk(0, 0, 0), s_d = mk(0, 0, 1), s_c = mk(0, 0, 0);
@@ -900,32 +938,14 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
                 if (depth < max_depth) {
                     // BSDF frame (bsdf.cc:14-21 from triangle.cc:127-155)
                     Bsdf b;
-                    const MatDev M = sc.mats[mbits & 0x7fffffffu];
-                    b.lobe = __float_as_int(M.a.w);
-                    b.type = lobe_type(b.lobe);
-                    b.R = xyz(M.a);
-                    b.T = xyz(M.b);
-                    b.etaB = M.b.w;
-                    b.eta = xyz(M.c);
-                    b.alpha = M.c.w;
-                    b.k = xyz(M.d);
-                    float3 dpdu, dpdv;
-                    coordinate_system(ng, dpdu, dpdv);
-                    b.n = ng;
-                    b.t = dpdu;
-                    b.b = dpdv;
+                    bsdf_material(sc, mbits, b);
+                    float3 ns = ng;
                     if (mbits >> 31) {
                         const float4* tn = sc.tri_n + 3 * k;
-                        float3 ns = normalize(add(add(mul(xyz(tn[0]), 1.0f - H.y - H.z), mul(xyz(tn[1]), H.y)),
-                                                  mul(xyz(tn[2]), H.z)));
-                        if (fabsf(dot(ns, ng)) < 1.0f - 1.0e-6f) {
-                            float3 ss = normalize(cross(ns, ng));
-                            float3 ts = normalize(cross(ns, ss));
-                            b.n = normalize(cross(ss, ts));
-                            b.t = ss;
-                            b.b = ts;
-                        }
+                        ns = normalize(add(add(mul(xyz(tn[0]), 1.0f - H.y - H.z), mul(xyz(tn[1]), H.y)),
+                                           mul(xyz(tn[2]), H.z)));
                     }
+                    bsdf_frame(ng, (mbits >> 31) != 0u, ns, b);
                     const float3 wo = neg(rd);
                     // NEE: uniformSampleOneLight + estimateDirectLight (mis.cc:21-136)
                     if (bsdf_has_nonspecular(b) && sc.num_lights > 0) {
@@ -989,7 +1009,15 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
                             }
                         }
                     }
-                    // Continuation (path.cc:83-94) and Russian roulette (path.cc:116-121)
+                    // Continuation (path.cc:83-94) and Russian roulette (path.cc:116-121); with a
+                    // guide the non-specular vertices go to the guide queue instead.
+                    if (GUIDED && guide_eligible(b)) {
+                        pushG = true;
+                        gq0 = make_float4(p.x, p.y, p.z, __uint_as_float(path));
+                        gq1 = make_float4(wo.x, wo.y, wo.z, __uint_as_float(flags));
+                        nA_beta = beta;
+                        gq3 = make_float4(ns.x, ns.y, ns.z, __int_as_float(k));
+                    } else {
                     float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
                     float pdf = 0.0f;
                     int st = 0;
@@ -1011,6 +1039,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
                             nA_flags = (flags & ~0x7fu) | (uint32_t)(depth + 1) | ((st & BX_SPECULAR) ? (1u << 6) : 0u);
                         }
                     }
+                    }
                 }
             }
             if (!black(Ladd)) {
@@ -1021,6 +1050,15 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
         const uint32_t sa = wave_append(pushA, &ps->countA[depth + 1]);
         const uint32_t ss = wave_append(pushS, &ps->countS[depth]);
         const uint32_t sm = wave_append(pushM, &ps->countM[depth]);
+        if (GUIDED) {
+            const uint32_t sg = wave_append(pushG, &ps->countG[depth]);
+            if (pushG) {
+                rb.qg[0][sg] = gq0;
+                rb.qg[1][sg] = gq1;
+                rb.qg[2][sg] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+                rb.qg[3][sg] = gq3;
+            }
+        }
         if (pushA) {
             rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
             rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
@@ -1109,6 +1147,159 @@ __global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int d
     }
 }
 
+// ---- guide: learned continuation proposal ---------------------------------------------------------
+// One persistent block of 512 threads per CU (the bf16 weights fill 58 KB of LDS once per
+// block).  Each wave takes 64 queued vertices: their inputs go through the MFMA network
+// together, then every lane samples its vertex's continuation from the one-sample MIS
+// mixture frac * guide + (1 - frac) * BSDF (path.cc:83-94 with the proposal added), applies
+// Russian roulette (path.cc:116-121) and, in training passes, records (inputs, w_i,
+// radiance so far, throughput) for the network.
+constexpr int kGuideBlock = 512;
+constexpr int kGuideObStride = 41;
+
+__device__ __forceinline__ uint32_t guide_dim(int depth) { return 4u + 8u * (uint32_t)(kMaxDepthCap + depth) + 1u; }
+
+__device__ __forceinline__ void guide_inputs(const GuideNetDev& net, float3 p, float3 n, float3 wo, float (&x)[7]) {
+    x[0] = clampf((p.x - net.lo[0]) * net.inv_ext[0], 0.0f, 1.0f);
+    x[1] = clampf((p.y - net.lo[1]) * net.inv_ext[1], 0.0f, 1.0f);
+    x[2] = clampf((p.z - net.lo[2]) * net.inv_ext[2], 0.0f, 1.0f);
+    float2 fn = dir_features(n.x, n.y, n.z), fw = dir_features(wo.x, wo.y, wo.z);
+    x[3] = fn.x;
+    x[4] = fn.y;
+    x[5] = fw.x;
+    x[6] = fw.y;
+}
+
+__global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
+                                                      GuideNetDev net, int depth) {
+    __shared__ __attribute__((aligned(16))) uint16_t w_lds[kGFwdSize];
+    __shared__ float b_lds[kGBiasSize];
+    __shared__ float obuf[kGuideBlock / 64][64 * kGuideObStride];
+    PassState* ps = rb.ps;
+    const uint32_t count = ps->countG[depth];
+    if (count == 0) return;
+    for (int i = threadIdx.x; i < kGFwdSize / 8; i += kGuideBlock)
+        reinterpret_cast<u32x4_t*>(w_lds)[i] = reinterpret_cast<const u32x4_t*>(net.wf)[i];
+    for (int i = threadIdx.x; i < kGBiasSize; i += kGuideBlock) b_lds[i] = net.bias[i];
+    __syncthreads();
+    const int kind = cfgp->sampler;
+    const uint32_t seed = cfgp->seed;
+    const int rr_depth = cfgp->rr_depth;
+    const uint32_t sample_base = ps->sample_base;
+    const int nxt = (depth & 1) ^ 1;
+    const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
+    const int lane = __lane_id(), wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    float* ob = obuf[wv];
+    const uint32_t waves = gridDim.x * (kGuideBlock / 64);
+    for (uint32_t base = (blockIdx.x * (kGuideBlock / 64) + wv) * 64u; base < count; base += waves * 64u) {
+        const uint32_t i = base + lane;
+        const bool active = i < count;
+        float4 G0 = make_float4(0, 0, 0, 0), G1 = make_float4(0, 0, 1, 0), G3 = make_float4(0, 0, 1, 0);
+        if (active) {
+            G0 = rb.qg[0][i];
+            G1 = rb.qg[1][i];
+            G3 = rb.qg[3][i];
+        }
+        float x[7];
+        guide_inputs(net, xyz(G0), xyz(G3), xyz(G1), x);
+#pragma unroll
+        for (int d = 0; d < 7; d++) ob[lane * 8 + d] = x[d];
+        wave_lds_sync();
+        f32x4_t out[3][4];
+        NoHook hook;
+        mlp_forward<4>(opaque(w_lds), opaque(b_lds), [&](int ct, int d) { return ob[(16 * ct + r) * 8 + d]; }, out,
+                       hook);
+        wave_lds_sync();
+#pragma unroll
+        for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int u = 16 * mt + 4 * g + k;
+                    if (u < kGOut) ob[(16 * ct + r) * kGuideObStride + u] = out[mt][ct][k];
+                }
+        wave_lds_sync();
+        bool pushA = false, pushR = false;
+        float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0), beta = mk(0, 0, 0);
+        uint32_t nA_flags = 0, path = 0, pixel = 0;
+        if (active) {
+            const float* o = ob + lane * kGuideObStride;
+            GuideMix mix;
+            mix_from_outputs([&](int u) { return o[u]; }, mix);
+            const float4 G2 = rb.qg[2][i];
+            path = __float_as_uint(G0.w);
+            pixel = __float_as_uint(G2.w);
+            beta = xyz(G2);
+            const uint32_t flags = __float_as_uint(G1.w);
+            const uint32_t sample = sample_base + (flags >> 7);
+            const int32_t k = __float_as_int(G3.w);
+            const float4 NG = sc.tri_ng[k];
+            const uint32_t mbits = __float_as_uint(NG.w);
+            const float3 ng = xyz(NG), p = xyz(G0), wo = xyz(G1);
+            Bsdf b;
+            bsdf_material(sc, mbits, b);
+            bsdf_frame(ng, (mbits >> 31) != 0u, xyz(G3), b);
+            const float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
+            const float us = sample_1d(kind, seed, pixel, sample, guide_dim(depth));
+            float3 wi = mk(0, 0, 1);
+            bool ok = true;
+            if (us < net.frac) {
+                wi = mix_sample(mix, us / net.frac, uc.x, uc.y);
+            } else {
+                float pdf_b = 0.0f;
+                int st = 0;
+                float3 fb = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf_b, BX_ALL, st);
+                ok = pdf_b > 0.0f && !black(fb);
+            }
+            if (ok) {
+                const float3 f = bsdf_f(b, wo, wi, BX_ALL);
+                const float pdf = (1.0f - net.frac) * bsdf_pdf(b, wo, wi, BX_ALL) +
+                                  net.frac * __expf(mix_log_pdf(mix, wi.x, wi.y, wi.z));
+                if (!black(f) && pdf > 0.0f) {
+                    float3 nb = cmul(beta, mul(f, fabsf(dot(wi, b.n)) / pdf));
+                    bool cont = true;
+                    if (depth > rr_depth) {
+                        float q = fminf(0.95f, gray(nb));
+                        if (sample_1d(kind, seed, pixel, sample, dim_base + 7) > q) cont = false;
+                        else nb = mul(nb, 1.0f / q);
+                    }
+                    if (cont) {
+                        pushA = true;
+                        if (net.record) {
+                            uint32_t h0 = path, h1 = (uint32_t)depth, h2 = sample, h3 = 0x7f4a7c15u;
+                            pcg4d(h0, h1, h2, h3);
+                            pushR = u32_to_unit(h0) < net.rec_keep;
+                        }
+                        nA_o = offset_origin(p, ng, wi);
+                        nA_d = wi;
+                        nA_beta = nb;
+                        nA_flags = (flags & ~0x7fu) | (uint32_t)(depth + 1);
+                    }
+                }
+            }
+        }
+        const uint32_t sa = wave_append(pushA, &ps->countA[depth + 1]);
+        if (pushA) {
+            rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
+            rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
+            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+        }
+        if (net.record) {
+            const uint32_t sr = wave_append(pushR, net.rec_count);
+            if (pushR && sr < net.rec_cap) {
+                const float4 L = rb.path_L[path];
+                net.rec[0][sr] = make_float4(x[0], x[1], x[2], x[3]);
+                net.rec[1][sr] = make_float4(x[4], x[5], x[6], __uint_as_float(path));
+                net.rec[2][sr] = make_float4(nA_d.x, nA_d.y, nA_d.z, 0.0f);
+                net.rec[3][sr] = make_float4(L.x, L.y, L.z, 0.0f);
+                net.rec[4][sr] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
 // ---- film: Kahan-compensated accumulation (film.cc:65-74) -------------------------------------------
 __device__ __forceinline__ void kahan(float& s, float& c, float x) {
     float y = x - c;
@@ -1141,12 +1332,14 @@ __global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg,
         rb.film_sqc[f] = qc;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && !ps->done) {
-        unsigned long long e = 0, sh = 0, mi = 0;
+        unsigned long long e = 0, sh = 0, mi = 0, gu = 0;
         for (int d = 0; d <= max_depth; d++) {
             e += ps->countA[d];
             sh += ps->countS[d];
             mi += ps->countM[d];
+            gu += ps->countG[d];
         }
+        ps->total_guided += gu;
         ps->total_paths += ps->n;
         ps->total_extend += e;
         ps->total_shadow += sh;
@@ -1175,6 +1368,28 @@ __global__ __launch_bounds__(256) void k_relmse(RenderBufs rb, uint32_t npix, co
         double I[3] = {(double)s.x * inv, (double)s.y * inv, (double)s.z * inv};
         for (int c = 0; c < 3; c++) {
             double R = ref[3 * (size_t)i + c];
+            double dd = I[c] - R;
+            acc += dd * dd / (R * R + eps);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_relmse_pixels(RenderBufs rb, int64_t n, const int32_t* __restrict__ pixels,
+                                                       const double* ref, double eps, double* partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float4 s = rb.film_sum[pixels[i]];
+        double inv = 1.0 / ((double)s.w + 1.0e-12);
+        double I[3] = {(double)s.x * inv, (double)s.y * inv, (double)s.z * inv};
+        for (int c = 0; c < 3; c++) {
+            double R = ref[3 * i + c];
             double dd = I[c] - R;
             acc += dd * dd / (R * R + eps);
         }
@@ -1271,9 +1486,17 @@ int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, 
     return SPK_OK;
 }
 
+int launch_relmse_pixels(const RenderBufs& b, int64_t n, const int32_t* pixels, const double* ref, double eps,
+                         double* out_partial, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_relmse_pixels, dim3(nblocks), dim3(256), 0, st, b, n, pixels, ref, eps, out_partial);
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, st, out_partial, nblocks, out_partial + nblocks);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
 template <int B>
 static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
-                         hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count) {
+                         hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
     hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(256), 0, st, b.cfg, b.ps);
     hipLaunchKernelGGL(k_camera, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
     for (int d = 0; d <= cfg.max_depth; d++) {
@@ -1283,10 +1506,15 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
         else
             hipLaunchKernelGGL((k_extend<B, kStackSlots, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
-        hipLaunchKernelGGL((k_shade<B>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+        if (guide)
+            hipLaunchKernelGGL((k_shade<B, true>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+        else
+            hipLaunchKernelGGL((k_shade<B, false>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
         if (d < cfg.max_depth) {
             hipLaunchKernelGGL((k_shadow<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
             hipLaunchKernelGGL((k_mis<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            // after the NEE kernels, so a training record sees this vertex's direct light
+            if (guide) hipLaunchKernelGGL(k_guide, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
         }
     }
     hipLaunchKernelGGL(k_film, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
@@ -1295,11 +1523,11 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
 }
 
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
-                hipEvent_t* ev_ext, int n_ev_ext, bool count) {
+                hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
     switch (ls.block) {
-    case 64: return launch_pass_b<64>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
-    case 256: return launch_pass_b<256>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
-    default: return launch_pass_b<128>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count);
+    case 64: return launch_pass_b<64>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    case 256: return launch_pass_b<256>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    default: return launch_pass_b<128>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
     }
 }
 

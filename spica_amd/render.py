@@ -119,6 +119,23 @@ class GpuRender:
                    "spk_render_relmse")
         return out.value
 
+    def relmse_pixels(self, pixels, ref, eps=1e-2):
+        """relMSE over the film pixels `pixels` (film indices) against ref [n, 3], reduced on the GPU in float64."""
+        pixels = np.ascontiguousarray(pixels, dtype=np.int32)
+        ref = np.ascontiguousarray(ref, dtype=np.float64)
+        assert ref.shape == (len(pixels), 3)
+        out = ctypes.c_double()
+        _abi.check(self.lib.spk_render_relmse_pixels(self.handle, len(pixels), pixels.ctypes.data_as(_abi.c_i32_p),
+                                                     ref.ctypes.data_as(_abi.c_d_p), eps, ctypes.byref(out)),
+                   "spk_render_relmse_pixels")
+        return out.value
+
+    def set_guide(self, guide):
+        """Attach a Guide (learned proposal, trained online) or None for plain BSDF sampling."""
+        self.guide = guide
+        _abi.check(self.lib.spk_render_set_guide(self.handle, guide.handle if guide is not None else None),
+                   "spk_render_set_guide")
+
     def stats(self):
         st = _abi.SpkRenderStats()
         _abi.check(self.lib.spk_render_stats_get(self.handle, ctypes.byref(st)), "spk_render_stats_get")
@@ -127,6 +144,95 @@ class GpuRender:
     def close(self):
         if self.handle:
             self.lib.spk_render_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def guide_opts(**kw):
+    o = _abi.SpkGuideOpts()
+    _abi.check(_abi.load().spk_guide_opts_default(ctypes.byref(o)), "spk_guide_opts_default")
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Guide:
+    """The learned path-guiding proposal (one-blob MLP -> 8-lobe vMF mixture) on the GPU."""
+
+    NPARAMS = 29480
+
+    def __init__(self, device=0, **opts):
+        self.lib = _abi.load()
+        self.opts = guide_opts(**opts)
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.spk_guide_create(device, ctypes.byref(self.opts), ctypes.byref(h)), "spk_guide_create")
+        self.handle = h
+
+    def params(self):
+        out = np.zeros(self.NPARAMS, dtype=np.float32)
+        _abi.check(self.lib.spk_guide_params_get(self.handle, out.ctypes.data_as(_abi.c_f_p)), "spk_guide_params_get")
+        return out
+
+    def set_params(self, th):
+        th = np.ascontiguousarray(th, dtype=np.float32)
+        assert th.shape == (self.NPARAMS,)
+        _abi.check(self.lib.spk_guide_params_set(self.handle, th.ctypes.data_as(_abi.c_f_p)), "spk_guide_params_set")
+
+    @staticmethod
+    def _f32(a, shape):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape == shape, (a.shape, shape)
+        return a
+
+    def eval(self, x, dirs):
+        n = len(x)
+        x, dirs = self._f32(x, (n, 7)), self._f32(dirs, (n, 3))
+        raw = np.zeros((n, 40), dtype=np.float32)
+        lp = np.zeros(n, dtype=np.float32)
+        _abi.check(self.lib.spk_guide_eval(self.handle, n, x.ctypes.data_as(_abi.c_f_p), dirs.ctypes.data_as(_abi.c_f_p),
+                                           raw.ctypes.data_as(_abi.c_f_p), lp.ctypes.data_as(_abi.c_f_p)),
+                   "spk_guide_eval")
+        return raw, lp
+
+    def sample(self, x, u):
+        n = len(x)
+        x, u = self._f32(x, (n, 7)), self._f32(u, (n, 3))
+        d = np.zeros((n, 3), dtype=np.float32)
+        pdf = np.zeros(n, dtype=np.float32)
+        _abi.check(self.lib.spk_guide_sample(self.handle, n, x.ctypes.data_as(_abi.c_f_p), u.ctypes.data_as(_abi.c_f_p),
+                                             d.ctypes.data_as(_abi.c_f_p), pdf.ctypes.data_as(_abi.c_f_p)),
+                   "spk_guide_sample")
+        return d, pdf
+
+    def grad(self, x, dirs, weight):
+        n = len(x)
+        x, dirs, weight = self._f32(x, (n, 7)), self._f32(dirs, (n, 3)), self._f32(weight, (n,))
+        g = np.zeros(self.NPARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _abi.check(self.lib.spk_guide_grad(self.handle, n, x.ctypes.data_as(_abi.c_f_p), dirs.ctypes.data_as(_abi.c_f_p),
+                                           weight.ctypes.data_as(_abi.c_f_p), g.ctypes.data_as(_abi.c_f_p),
+                                           ctypes.byref(loss)), "spk_guide_grad")
+        return loss.value, g
+
+    def train(self, x, dirs, weight, steps=1):
+        n = len(x)
+        x, dirs, weight = self._f32(x, (n, 7)), self._f32(dirs, (n, 3)), self._f32(weight, (n,))
+        _abi.check(self.lib.spk_guide_train(self.handle, n, x.ctypes.data_as(_abi.c_f_p), dirs.ctypes.data_as(_abi.c_f_p),
+                                            weight.ctypes.data_as(_abi.c_f_p), int(steps)), "spk_guide_train")
+
+    def stats(self):
+        st = _abi.SpkGuideStats()
+        _abi.check(self.lib.spk_guide_stats_get(self.handle, ctypes.byref(st)), "spk_guide_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.spk_guide_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
