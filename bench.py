@@ -8,8 +8,11 @@ over the whole frame (2,073,600 paths), i.e. one pass of the hot path; the full
 
 Printed (one JSON line): total traced rays per second (camera + continuation +
 shadow + MIS rays, all traversals), the closest-hit traversal kernel against the
-HBM roofline, and spica's CPU path (the FP64 C restatement, the reference's
-estimator and BVH) timed on this host's cores on a bounded sample.
+HBM roofline, the relMSE of the rendered C1 image against its 65536-spp float64 CPU
+reference, the wall time to relMSE 1e-3 of plain and guided tracing on the
+occluded-glossy scene (C3, online-trained proposal included in the guided time), and
+spica's CPU path (the FP64 C restatement, the reference's estimator and BVH) timed
+on this host's cores on a bounded sample.
 """
 import argparse
 import json
@@ -72,6 +75,53 @@ def cpu_baseline(w, threads, budget_s):
                 mpaths_per_s=paths / elapsed / 1e6)
 
 
+def load_reference(name):
+    """65536-spp float64 CPU reference on a stratified pixel subset (tests/golden/make_reference.py)."""
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", f"{name}_ref.npz")
+    if not os.path.exists(path):
+        return None
+    d = np.load(path)
+    return d["pixels"], d["mean"], int(d["spp"])
+
+
+def time_to_relmse(gs, w, ref, guide_cfg, target, max_spp, queue, block, chunk=32):
+    """Wall time (render calls only, synchronised) until relMSE against the reference
+    falls to `target`, checked every `chunk` samples per pixel on the GPU in float64."""
+    from spica_amd import render
+
+    pix, refm, _ = ref
+    gr = render.GpuRender(gs, w.camera, w.render_desc(spp=max_spp), queue_size=queue, block_size=block)
+    g = None
+    if guide_cfg is not None:
+        g = render.Guide(**guide_cfg)
+        gr.set_guide(g)
+    t = 0.0
+    spp = 0
+    e = float("inf")
+    hit = None
+    trace = []
+    while spp < max_spp:
+        k = min(chunk, max_spp - spp)
+        t0 = time.perf_counter()
+        gr.run(k, sync=True)
+        t += time.perf_counter() - t0
+        spp += k
+        e = gr.relmse_pixels(pix, refm)
+        if spp in (64, 256, 1024) or e <= target:
+            trace.append([spp, round(t, 4), float(f"{e:.4g}")])
+        if e <= target:
+            hit = (spp, t)
+            break
+    st = gr.stats()
+    gr.close()
+    if g is not None:
+        g.close()
+    return dict(seconds=None if hit is None else round(hit[1], 4), spp=None if hit is None else hit[0],
+                last_relmse=float(f"{e:.4g}"), trace=trace, guided_vertices=st["guided_vertices"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +132,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queue", type=int, default=0, help="paths in flight (0 = autotune)")
     ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -156,6 +207,42 @@ def main():
     log(f"bench: {args.steps} steps in {dt*1e3:.1f} ms: {value:.1f} Mrays/s, {paths/dt/1e6:.1f} Mpaths/s; "
         f"closest-hit {ext_rays} rays in {ext_ms:.1f} ms over {ext_launches} launches")
 
+    c1_ref = load_reference("c1")
+    c1_relmse = None
+    if c1_ref is not None:
+        c1_relmse = dict(value=float(f"{gr.relmse_pixels(c1_ref[0], c1_ref[1]):.5g}"),
+                         spp=args.warmup + args.steps,
+                         reference=f"{c1_ref[2]}-spp float64 CPU render, {len(c1_ref[0])} stratified pixels")
+        log(f"bench: C1 relMSE at {args.warmup + args.steps} spp: {c1_relmse['value']}")
+    gr.close()
+
+    ttr = None
+    c3_ref = load_reference("c3")
+    if not args.no_guide and c3_ref is not None:
+        w3 = configs.c3_occluded_glossy()
+        gs3 = render.GpuScene(w3.scene)
+        guide_cfg = dict(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=2, learning_rate=1e-2,
+                         guide_fraction=0.5)
+        warm = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=16), queue_size=args.queue, block_size=args.block)
+        warm.run(16)
+        warm.close()
+        log("bench: C3 time to relMSE 1e-3, plain")
+        plain = time_to_relmse(gs3, w3, c3_ref, None, 1e-3, 16384, args.queue, args.block)
+        log(f"bench: plain {plain}")
+        log("bench: C3 time to relMSE 1e-3, guided")
+        guided = time_to_relmse(gs3, w3, c3_ref, guide_cfg, 1e-3, 16384, args.queue, args.block)
+        log(f"bench: guided {guided}")
+        ratio = None
+        if plain["seconds"] and guided["seconds"]:
+            ratio = round(guided["seconds"] / plain["seconds"], 4)
+        ttr = dict(workload="C3 occluded glossy: seed-7 sphere field (60% GGX alpha 0.1-0.3), light inside a box open "
+                            "towards the ceiling, 1280x720, maxDepth 16, Sobol seed 7",
+                   target=1e-3, eps=1e-2,
+                   reference=f"{c3_ref[2]}-spp float64 CPU render (spica's estimator), {len(c3_ref[0])} stratified pixels",
+                   plain=plain, guided=guided, guided_over_plain=ratio,
+                   guide=dict(guide_cfg, network="one-blob(7x32) -> 64x4 ReLU -> 8 vMF lobes, bf16 MFMA, fp32 accum"))
+        gs3.close()
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_threads} threads for ~{args.cpu_seconds:.0f} s")
@@ -193,10 +280,11 @@ def main():
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "rays_per_launch": round(ext_rays / max(1, ext_launches), 1),
         },
+        "relmse": c1_relmse,
+        "time_to_relmse": ttr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    gr.close()
     gs.close()
 
 

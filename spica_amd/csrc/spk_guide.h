@@ -56,7 +56,12 @@ __host__ __device__ inline uint32_t bf16_bits(float f) {
     memcpy(&u, &f, 4);
     return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
 }
-__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// One v_cvt_pk_bf16_f32 (round to nearest even, as bf16_bits).
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{lo, hi}, bf16x2_t));
+}
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
@@ -183,12 +188,14 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
         }
 }
 
-// Hides a pointer's value from the optimiser for one loop iteration, so per-sample loops
-// reload weights instead of hoisting every operand into registers (which spills).
+// Offsets a pointer by a zero the optimiser cannot see through, once per loop iteration,
+// so per-sample loops reload weights instead of hoisting every operand into registers
+// (which spills).  The base keeps its address space (LDS stays ds_read).
 template <class T>
 __device__ __forceinline__ T* opaque(T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return p + z;
 }
 
 // Orders LDS traffic between the lanes of one wave (no block barrier: waves of a
