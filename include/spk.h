@@ -195,6 +195,49 @@ int spk_guide_stats_get(spk_guide* guide, spk_guide_stats* out);
  * serves one render at a time. */
 int spk_render_set_guide(spk_render* render, spk_guide* guide);
 
+/* ---- Batched rendering ---------------------------------------------------------------
+ * Many images, each with its own scene, camera, sample count and seed, rendered by one
+ * set of wavefront queues: their work is cut into chunks (k samples of a pixel range)
+ * packed into shared passes, so small images fill the queues together, and a pass is one
+ * captured hipGraph replayed (or the whole batch one graph).  Each image's film equals
+ * the film spk_render_run gives for it alone, bit for bit (samples are keyed by pixel,
+ * sample and dimension, and every pixel accumulates in sample order).  Replaces running
+ * spica's PathIntegrator::render once per scene file (sources/integrators/path/path.cc,
+ * sources/core/integrator.cc:62-107).  Limits: W * H <= 2^20 per image, up to 4096
+ * images, no environment-map or guide per batch image. */
+typedef struct spk_batch spk_batch;
+
+typedef struct spk_batch_image {
+    spk_scene* scene;
+    spk_camera_desc camera;
+    int32_t spp;
+    uint32_t seed;
+} spk_batch_image;
+
+typedef struct spk_batch_desc {
+    int32_t num_images;
+    const spk_batch_image* images;
+    int32_t max_depth;
+    int32_t rr_depth;
+    int32_t sampler;      /* SPK_SAMPLER_INDEPENDENT or SPK_SAMPLER_SOBOL */
+    int32_t graph;        /* 0: plain launches, 1: one graph per pass, replayed, 2: the whole batch one graph */
+} spk_batch_desc;
+
+int spk_batch_create(const spk_batch_desc* desc, const spk_render_opts* opts, spk_batch** out);
+int spk_batch_destroy(spk_batch* batch);
+/* Renders every image to its sample count (films are cleared first). */
+int spk_batch_run(spk_batch* batch, int32_t sync);
+int spk_batch_film(spk_batch* batch, int32_t image, double* sum_rgb, double* sumsq_rgb, double* count);
+typedef struct spk_batch_info {
+    int64_t passes;       /* passes per run */
+    int64_t chunks;
+    int64_t paths;        /* camera paths per run */
+    int64_t film_pixels;
+    int64_t queue;        /* paths in flight */
+} spk_batch_info;
+int spk_batch_info_get(spk_batch* batch, spk_batch_info* out);
+int spk_batch_stats_get(spk_batch* batch, spk_render_stats* out);
+
 /* Raw 32-bit words of the counter-based sample streams for n (pixel, sample, dim)
  * keys (host arrays).  on_device = 0 evaluates on the host, 1 on the GPU with the
  * same code the kernels use.  Replaces Sampler::get1D (sources/core/sampler.h:28-31,

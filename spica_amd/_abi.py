@@ -132,6 +132,36 @@ class SpkRenderStats(ctypes.Structure):
     ]
 
 
+class SpkBatchImage(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_void_p),
+        ("camera", SpkCameraDesc),
+        ("spp", ctypes.c_int32),
+        ("seed", ctypes.c_uint32),
+    ]
+
+
+class SpkBatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_images", ctypes.c_int32),
+        ("images", ctypes.POINTER(SpkBatchImage)),
+        ("max_depth", ctypes.c_int32),
+        ("rr_depth", ctypes.c_int32),
+        ("sampler", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+    ]
+
+
+class SpkBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("passes", ctypes.c_int64),
+        ("chunks", ctypes.c_int64),
+        ("paths", ctypes.c_int64),
+        ("film_pixels", ctypes.c_int64),
+        ("queue", ctypes.c_int64),
+    ]
+
+
 class SpkGuideOpts(ctypes.Structure):
     _fields_ = [
         ("seed", ctypes.c_uint32),
@@ -190,6 +220,13 @@ SIGNATURES = {
     "spk_guide_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, ctypes.c_int32]),
     "spk_guide_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkGuideStats)]),
     "spk_render_set_guide": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "spk_batch_create": (ctypes.c_int, [ctypes.POINTER(SpkBatchDesc), ctypes.POINTER(SpkRenderOpts),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "spk_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "spk_batch_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_d_p, c_d_p, c_d_p]),
+    "spk_batch_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkBatchInfo)]),
+    "spk_batch_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkRenderStats)]),
 }
 
 _lib = None

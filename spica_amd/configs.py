@@ -8,6 +8,9 @@ C1  Sphere field: seed-1 generator, 512 spheres of 640 triangles (UV sphere 20 x
     centres U[-5,5]^3 and radius log-U[0.05,0.4] on a 20x20 ground quad; 40% Lambert
     U[0.1,0.9], 20% mirror, 20% dielectric IOR 1.5, 20% GGX alpha U[0.05,0.5]; two
     quad area lights; 1920x1080, 256 spp, maxDepth 16, Sobol.
+C2  Batch: 64 images; image i draws its scene from the C1 generator with seed i and
+    U{16..2048} spheres (10k-1.3M triangles), a W x H film with W, H ~ U{128..1024} and
+    U{16..512} samples per pixel; maxDepth 16, Sobol seed i.
 C3  Occluded glossy: the same generator with seed 7, 60% GGX alpha U[0.1,0.3], one
     quad light inside a box open on its top that lights the scene via the ceiling;
     1280x720, 1024 spp, maxDepth 16.
@@ -115,6 +118,32 @@ def c1_spheres(width=1920, height=1080, spp=256, max_depth=16, seed=1, n_spheres
     s, _ = sphere_field(seed, n_spheres)
     cam = Camera(width, height, look_at((0.0, 3.0, -18.0), (0.0, -1.0, 0.0), (0.0, 1.0, 0.0)), 40.0)
     return Workload("C1 sphere field", s, cam, spp, max_depth, _abi.SPK_SAMPLER_SOBOL, seed)
+
+
+class BatchWorkload:
+    def __init__(self, name, items, max_depth, sampler):
+        self.name = name
+        self.items = items          # list of (Scene, Camera, spp, seed)
+        self.max_depth = max_depth
+        self.sampler = sampler
+
+    def paths(self):
+        return sum(c.width * c.height * spp for _, c, spp, _ in self.items)
+
+
+def c2_batch(n_images=64, draw_seed=2, max_depth=16, spheres=(16, 2048), res=(128, 1024), spp=(16, 512)):
+    """The batch workload: per image an independent sphere field, film size and sample count."""
+    rng = np.random.default_rng(draw_seed)
+    items = []
+    for i in range(n_images):
+        n = int(rng.integers(spheres[0], spheres[1] + 1))
+        w = int(rng.integers(res[0], res[1] + 1))
+        h = int(rng.integers(res[0], res[1] + 1))
+        k = int(rng.integers(spp[0], spp[1] + 1))
+        sc, _ = sphere_field(seed=i, n_spheres=n)
+        cam = Camera(w, h, look_at((0.0, 3.0, -18.0), (0.0, -1.0, 0.0), (0.0, 1.0, 0.0)), 40.0)
+        items.append((sc, cam, k, i))
+    return BatchWorkload(f"C2 batch of {n_images}", items, max_depth, _abi.SPK_SAMPLER_SOBOL)
 
 
 def c3_occluded_glossy(width=1280, height=720, spp=1024, max_depth=16, seed=7, n_spheres=512):

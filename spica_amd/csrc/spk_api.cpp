@@ -229,6 +229,22 @@ struct spk_scene {
     spk_scene_stats stats;
 };
 
+// PerspectiveCamera (cameras/perspective.cc:20-51) as the kernels use it.
+static void camera_dev(const spk_camera_desc* cam, CameraDev& d) {
+    memset(&d, 0, sizeof(d));
+    const double PI = 3.14159265358979323846;
+    double tanHalf = tan(cam->fov_deg * PI / 180.0 / 2.0);
+    double aspect = (double)cam->width / cam->height;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 4; j++) d.m[i * 4 + j] = (float)cam->to_world[i * 4 + j];
+    d.sx_scale = (float)(aspect * tanHalf);
+    d.sy_scale = (float)tanHalf;
+    d.lens_r = (float)cam->aperture_radius;
+    d.focal = (float)cam->focus_distance;
+    d.W = cam->width;
+    d.H = cam->height;
+}
+
 struct spk_render {
     spk_scene* scene = nullptr;
     int device = 0;
@@ -246,6 +262,7 @@ struct spk_render {
     int64_t extend_launches = 0;
     int64_t extend_rays_timed = 0;
     std::vector<hipEvent_t> events;
+    hipGraphExec_t pass_exec = nullptr;  // use_graph: one captured plain pass, replayed
     // learned proposal
     spk_guide* guide = nullptr;
     DevBuf qg[4], rec[5];
@@ -658,17 +675,7 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
 
     ConfigDev& c = r->cfg;
     memset(&c, 0, sizeof(c));
-    const double PI = 3.14159265358979323846;
-    double tanHalf = tan(cam->fov_deg * PI / 180.0 / 2.0);
-    double aspect = (double)cam->width / cam->height;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 4; j++) c.cam.m[i * 4 + j] = (float)cam->to_world[i * 4 + j];
-    c.cam.sx_scale = (float)(aspect * tanHalf);
-    c.cam.sy_scale = (float)tanHalf;
-    c.cam.lens_r = (float)cam->aperture_radius;
-    c.cam.focal = (float)cam->focus_distance;
-    c.cam.W = cam->width;
-    c.cam.H = cam->height;
+    camera_dev(cam, c.cam);
     c.W = cam->width;
     c.H = cam->height;
     c.npix = (uint32_t)cam->width * (uint32_t)cam->height;
@@ -753,6 +760,7 @@ int spk_render_destroy(spk_render* r) {
         (void)hipStreamDestroy(r->stream);
     }
     for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    if (r->pass_exec) (void)hipGraphExecDestroy(r->pass_exec);
     DevBuf* bufs[] = {&r->qa_o[0], &r->qa_o[1], &r->qa_d[0], &r->qa_d[1], &r->qa_b[0], &r->qa_b[1], &r->hit,
                       &r->qs_o, &r->qs_d, &r->qs_c, &r->qm_o, &r->qm_d, &r->qm_c, &r->path_L, &r->film_sum,
                       &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch,
@@ -808,6 +816,23 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
             SPK_CHECK_HIP(hipEventCreate(&e));
             r->events.push_back(e);
         }
+    }
+    if (r->opts.use_graph && !r->opts.profile && !net) {
+        // The pass reads its sample range from device memory, so one capture serves every pass.
+        if (!r->pass_exec) {
+            hipGraph_t g;
+            SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            int crc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, nullptr, 0, r->opts.count_visits != 0,
+                                  nullptr);
+            hipError_t e = hipStreamEndCapture(r->stream, &g);
+            if (crc) return crc;
+            SPK_CHECK_HIP(e);
+            e = hipGraphInstantiate(&r->pass_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            SPK_CHECK_HIP(e);
+        }
+        for (int64_t p = 0; p < passes; p++) SPK_CHECK_HIP(hipGraphLaunch(r->pass_exec, r->stream));
+        return SPK_OK;
     }
     for (int64_t p = 0; p < passes; p++) {
         hipEvent_t* ev = r->opts.profile ? &r->events[(size_t)p * nev] : nullptr;
@@ -1062,6 +1087,277 @@ int spk_sampler_u32(int32_t kind, uint32_t seed, int64_t n, const uint32_t* pixe
     c.release();
     o.release();
     return SPK_OK;
+}
+
+// ---- batched rendering ---------------------------------------------------------------------
+
+}  // extern "C"
+
+struct spk_batch {
+    spk_render* r = nullptr;          // queues, pass state, stream; its film holds every image
+    std::vector<spk_scene*> scenes;
+    std::vector<ImageDev> images;
+    std::vector<ChunkDev> chunks;
+    std::vector<PassDev> passes;
+    DevBuf scenes_dev, images_dev, chunks_dev, passes_dev;
+    int64_t paths = 0;
+    int64_t film_pixels = 0;
+    int graph = 0;
+    hipGraphExec_t exec = nullptr;    // one pass (graph = 1) or the whole batch (graph = 2)
+};
+
+namespace {
+
+// Work of image `img` as chunks of whole-frame sample groups, or pixel ranges at one
+// sample each when the frame exceeds the queue.
+void batch_chunks(uint32_t img, uint32_t npix, uint32_t spp, uint32_t cap, std::vector<ChunkDev>& out) {
+    if (npix <= cap) {
+        const uint32_t k = std::min<uint32_t>({spp, cap / npix, 1u << 14});
+        for (uint32_t s0 = 0; s0 < spp; s0 += k) {
+            ChunkDev c = {img, 0u, npix, s0, std::min(k, spp - s0), 0u, 0u, 0u};
+            out.push_back(c);
+        }
+    } else {
+        for (uint32_t s0 = 0; s0 < spp; s0++)
+            for (uint32_t p0 = 0; p0 < npix; p0 += cap) {
+                ChunkDev c = {img, p0, std::min(cap, npix - p0), s0, 1u, 0u, 0u, 0u};
+                out.push_back(c);
+            }
+    }
+}
+
+// Packs chunks in order into passes of at most `cap` paths, at most one chunk per image
+// per pass (two chunks of one image in a pass could touch the same film pixel).
+void batch_passes(std::vector<ChunkDev>& ch, uint32_t cap, std::vector<PassDev>& out) {
+    size_t i = 0;
+    while (i < ch.size()) {
+        PassDev p = {(uint32_t)i, (uint32_t)i, 0u, 0u};
+        std::vector<uint32_t> imgs;
+        while (i < ch.size()) {
+            ChunkDev& c = ch[i];
+            const uint32_t paths = c.n_pix * c.k;
+            if (p.n_paths + paths > cap) break;
+            if (std::find(imgs.begin(), imgs.end(), c.img) != imgs.end()) break;
+            c.path_off = p.n_paths;
+            c.pix_off = p.n_pixels;
+            p.n_paths += paths;
+            p.n_pixels += c.n_pix;
+            imgs.push_back(c.img);
+            i++;
+        }
+        p.chunk_end = (uint32_t)i;
+        out.push_back(p);
+    }
+}
+
+int batch_launch_all(spk_batch* b) {
+    spk_render* r = b->r;
+    int rc;
+    if ((rc = launch_clear_film(r->bufs, (uint32_t)b->film_pixels, r->stream))) return rc;
+    if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
+    for (size_t p = 0; p < b->passes.size(); p++)
+        if ((rc = launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream))) return rc;
+    return SPK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_batch** out) {
+    clear_error();
+    if (!d || !out || d->num_images <= 0 || !d->images) {
+        set_error("spk_batch_create: no images");
+        return SPK_ERR_INVALID;
+    }
+    *out = nullptr;
+    if ((uint32_t)d->num_images > kBatchMaxImages) {
+        set_error("spk_batch_create: %d images, at most %u", d->num_images, kBatchMaxImages);
+        return SPK_ERR_INVALID;
+    }
+    if (d->graph < 0 || d->graph > 2) {
+        set_error("spk_batch_create: graph mode %d not 0, 1 or 2", d->graph);
+        return SPK_ERR_INVALID;
+    }
+    std::unique_ptr<spk_batch> b(new spk_batch());
+    b->graph = d->graph;
+    int64_t film = 0, spill_cap = 0;
+    const int device = d->images[0].scene ? d->images[0].scene->device : 0;
+    for (int i = 0; i < d->num_images; i++) {
+        const spk_batch_image& im = d->images[i];
+        const spk_camera_desc& c = im.camera;
+        if (!im.scene || im.scene->device != device) {
+            set_error("spk_batch_create: image %d has no scene or a scene on another device", i);
+            return SPK_ERR_INVALID;
+        }
+        if (c.width <= 0 || c.height <= 0 || (int64_t)c.width * c.height > (int64_t(1) << kBatchPixelBits)) {
+            set_error("spk_batch_create: image %d film %dx%d outside 1 .. 2^20 pixels", i, c.width, c.height);
+            return SPK_ERR_INVALID;
+        }
+        if (c.to_world[12] != 0.0 || c.to_world[13] != 0.0 || c.to_world[14] != 0.0 || c.to_world[15] != 1.0) {
+            set_error("spk_batch_create: image %d camera toWorld must be affine", i);
+            return SPK_ERR_INVALID;
+        }
+        if (im.spp <= 0 || im.spp >= (1 << 25)) {
+            set_error("spk_batch_create: image %d spp %d outside [1, 2^25)", i, im.spp);
+            return SPK_ERR_INVALID;
+        }
+        if (im.scene->dev.env.present) {
+            set_error("spk_batch_create: image %d scene has an environment map (not batched)", i);
+            return SPK_ERR_INVALID;
+        }
+        ImageDev id;
+        memset(&id, 0, sizeof(id));
+        camera_dev(&c, id.cam);
+        id.seed = im.seed;
+        id.film_off = (uint32_t)film;
+        film += (int64_t)c.width * c.height;
+        if (film >= (int64_t(1) << 31)) {
+            set_error("spk_batch_create: films exceed 2^31 pixels");
+            return SPK_ERR_INVALID;
+        }
+        b->images.push_back(id);
+        b->scenes.push_back(im.scene);
+        spill_cap = std::max<int64_t>(spill_cap, im.scene->spill_cap);
+    }
+    b->film_pixels = film;
+    // The queues, pass state and film come from a render of a 1-row "film" holding every
+    // image's pixels; its camera is never used by the batch kernels.
+    spk_camera_desc fc;
+    memset(&fc, 0, sizeof(fc));
+    fc.width = (int32_t)film;
+    fc.height = 1;
+    fc.fov_deg = 45.0;
+    fc.to_world[0] = fc.to_world[5] = fc.to_world[10] = fc.to_world[15] = 1.0;
+    spk_render_desc rd;
+    memset(&rd, 0, sizeof(rd));
+    rd.spp = 1;
+    rd.max_depth = d->max_depth;
+    rd.rr_depth = d->rr_depth;
+    rd.sampler = d->sampler;
+    int rc;
+    spk_scene* s0 = b->scenes[0];
+    const int s0_cap = s0->spill_cap;
+    s0->spill_cap = (int)spill_cap;  // size the traversal spill for the deepest BVH
+    rc = spk_render_create(s0, &fc, &rd, opts, &b->r);
+    s0->spill_cap = s0_cap;
+    if (rc) return rc;
+    spk_render* r = b->r;
+    const uint32_t cap = r->capacity;
+    for (int i = 0; i < d->num_images; i++) {
+        const spk_camera_desc& c = d->images[i].camera;
+        batch_chunks((uint32_t)i, (uint32_t)(c.width * c.height), (uint32_t)d->images[i].spp, cap, b->chunks);
+        b->paths += (int64_t)c.width * c.height * d->images[i].spp;
+    }
+    batch_passes(b->chunks, cap, b->passes);
+    std::vector<SceneDev> sd;
+    for (spk_scene* sc : b->scenes) sd.push_back(sc->dev);
+    if ((rc = b->scenes_dev.upload(sd.data(), sd.size() * sizeof(SceneDev))) ||
+        (rc = b->images_dev.upload(b->images.data(), b->images.size() * sizeof(ImageDev))) ||
+        (rc = b->chunks_dev.upload(b->chunks.data(), b->chunks.size() * sizeof(ChunkDev))) ||
+        (rc = b->passes_dev.upload(b->passes.data(), b->passes.size() * sizeof(PassDev)))) {
+        spk_render_destroy(r);
+        return rc;
+    }
+    r->bufs.scenes = b->scenes_dev.as<SceneDev>();
+    r->bufs.images = b->images_dev.as<ImageDev>();
+    r->bufs.chunks = b->chunks_dev.as<ChunkDev>();
+    r->bufs.passes = b->passes_dev.as<PassDev>();
+    r->bufs.num_passes = (uint32_t)b->passes.size();
+    r->bufs.spill_cap = (int)spill_cap;
+    SPK_CHECK_HIP(hipMemcpy(r->cfg_dev.p, &r->cfg, sizeof(ConfigDev), hipMemcpyHostToDevice));
+    if (b->graph) {
+        // Captured work: kernel launches only (no memset nodes).
+        hipGraph_t g;
+        SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        int crc = b->graph == 2 ? batch_launch_all(b.get()) : launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream);
+        hipError_t e = hipStreamEndCapture(r->stream, &g);
+        if (crc) return crc;
+        SPK_CHECK_HIP(e);
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SPK_CHECK_HIP(e);
+    }
+    *out = b.release();
+    return SPK_OK;
+}
+
+int spk_batch_destroy(spk_batch* b) {
+    clear_error();
+    if (!b) return SPK_OK;
+    if (b->r) (void)hipSetDevice(b->r->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    DevBuf* bufs[] = {&b->scenes_dev, &b->images_dev, &b->chunks_dev, &b->passes_dev};
+    for (DevBuf* x : bufs) x->release();
+    if (b->r) spk_render_destroy(b->r);
+    delete b;
+    return SPK_OK;
+}
+
+int spk_batch_run(spk_batch* b, int32_t sync) {
+    clear_error();
+    if (!b) return SPK_ERR_INVALID;
+    spk_render* r = b->r;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    int rc;
+    if (b->graph == 2) {
+        SPK_CHECK_HIP(hipGraphLaunch(b->exec, r->stream));
+    } else if (b->graph == 1) {
+        if ((rc = launch_clear_film(r->bufs, (uint32_t)b->film_pixels, r->stream))) return rc;
+        if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
+        for (size_t p = 0; p < b->passes.size(); p++) SPK_CHECK_HIP(hipGraphLaunch(b->exec, r->stream));
+    } else if ((rc = batch_launch_all(b))) {
+        return rc;
+    }
+    r->samples_done++;
+    if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    return SPK_OK;
+}
+
+int spk_batch_film(spk_batch* b, int32_t image, double* sum_rgb, double* sumsq_rgb, double* count) {
+    clear_error();
+    if (!b || image < 0 || image >= (int32_t)b->images.size()) return SPK_ERR_INVALID;
+    spk_render* r = b->r;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+    const ImageDev& im = b->images[image];
+    const size_t P = (size_t)im.cam.W * im.cam.H, off = im.film_off;
+    std::vector<float4> s(P), c(P), q(P), qc(P);
+    SPK_CHECK_HIP(hipMemcpy(s.data(), r->film_sum.as<float4>() + off, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(c.data(), r->film_c.as<float4>() + off, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(q.data(), r->film_sq.as<float4>() + off, P * 16, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(qc.data(), r->film_sqc.as<float4>() + off, P * 16, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; i++) {
+        if (sum_rgb) {
+            sum_rgb[3 * i] = (double)s[i].x - (double)c[i].x;
+            sum_rgb[3 * i + 1] = (double)s[i].y - (double)c[i].y;
+            sum_rgb[3 * i + 2] = (double)s[i].z - (double)c[i].z;
+        }
+        if (sumsq_rgb) {
+            sumsq_rgb[3 * i] = (double)q[i].x - (double)qc[i].x;
+            sumsq_rgb[3 * i + 1] = (double)q[i].y - (double)qc[i].y;
+            sumsq_rgb[3 * i + 2] = (double)q[i].z - (double)qc[i].z;
+        }
+        if (count) count[i] = (double)s[i].w;
+    }
+    return SPK_OK;
+}
+
+int spk_batch_info_get(spk_batch* b, spk_batch_info* out) {
+    clear_error();
+    if (!b || !out) return SPK_ERR_INVALID;
+    out->passes = (int64_t)b->passes.size();
+    out->chunks = (int64_t)b->chunks.size();
+    out->paths = b->paths;
+    out->film_pixels = b->film_pixels;
+    out->queue = b->r->capacity;
+    return SPK_OK;
+}
+
+int spk_batch_stats_get(spk_batch* b, spk_render_stats* out) {
+    clear_error();
+    if (!b) return SPK_ERR_INVALID;
+    return spk_render_stats_get(b->r, out);
 }
 
 }  // extern "C"

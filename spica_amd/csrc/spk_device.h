@@ -78,6 +78,30 @@ struct CameraDev {
 
 static constexpr int kMaxDepthCap = 64;
 
+// ---- batched rendering (many images, each its own scene, in shared passes) ----
+// Queue entries of a batch carry pixel | image << kBatchPixelBits in their pixel field
+// and the absolute sample index in flag bits 7..31.
+static constexpr uint32_t kBatchPixelBits = 20;
+static constexpr uint32_t kBatchMaxImages = 1u << (32 - kBatchPixelBits);
+
+struct ImageDev {
+    CameraDev cam;          // cam.W x cam.H film
+    uint32_t seed;
+    uint32_t film_off;      // first film pixel of the image
+    uint32_t pad[2];
+};
+
+// k samples of the pixel-order range [pixel_base, pixel_base + n_pix) of one image; its
+// paths are [path_off, path_off + k n_pix) and its film pixels [pix_off, pix_off + n_pix)
+// of the pass (sample-major, as single-image passes).
+struct ChunkDev {
+    uint32_t img, pixel_base, n_pix, sample_base, k, path_off, pix_off, pad;
+};
+
+struct PassDev {
+    uint32_t chunk_begin, chunk_end, n_paths, n_pixels;
+};
+
 struct ConfigDev {
     CameraDev cam;
     int W, H;
@@ -103,6 +127,10 @@ struct PassState {
     uint32_t n_pix;         // pixel slots per sample in this pass
     uint32_t k_samples;     // samples in this pass
     uint32_t pad;
+    uint32_t chunk_begin;   // batch: chunks of this pass
+    uint32_t chunk_end;
+    uint32_t n_pixels;      // batch: film pixels touched by this pass
+    uint32_t pad2;
     uint32_t countA[kMaxDepthCap + 2];
     uint32_t countS[kMaxDepthCap + 2];
     uint32_t countM[kMaxDepthCap + 2];

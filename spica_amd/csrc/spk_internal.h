@@ -98,6 +98,12 @@ struct RenderBufs {
     int spill_cap;
     ConfigDev* cfg;     // device copy
     PassState* ps;      // device pass state
+    // batched rendering only
+    const SceneDev* scenes;   // per image
+    const ImageDev* images;
+    const ChunkDev* chunks;
+    const PassDev* passes;
+    uint32_t num_passes;
 };
 
 struct LaunchShape {
@@ -109,6 +115,8 @@ struct LaunchShape {
 
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls,
                 hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide);
+// One pass of a batch (images, chunks and passes in b); scenes come from b.scenes.
+int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls, hipStream_t st);
 int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st);
 int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st);
 int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial,

@@ -298,35 +298,52 @@ __global__ void k_set_pass(PassState* ps, uint32_t index) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ps->pass_index = index;
 }
 
-__global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps) {
+template <bool BATCH>
+__global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps, const PassDev* __restrict__ passes,
+                            uint32_t num_passes) {
     // One block: thread 0 computes the pass, all threads clear the counters.
     __shared__ uint32_t n_sh;
     if (threadIdx.x == 0) {
         uint32_t j = ps->pass_index;
-        uint32_t total = cfg->end_sample - cfg->first_sample;
-        uint32_t sample_base, pixel_base, n_pix, k;
-        bool done;
-        if (cfg->chunks > 1) {
-            uint32_t s = j / cfg->chunks, c = j % cfg->chunks;
-            done = s >= total;
-            sample_base = cfg->first_sample + s;
-            pixel_base = c * cfg->queue;
-            n_pix = min(cfg->queue, cfg->npix - pixel_base);
-            k = 1;
+        if (BATCH) {
+            const bool done = j >= num_passes;
+            PassDev P = {0u, 0u, 0u, 0u};
+            if (!done) P = passes[j];
+            ps->done = done ? 1u : 0u;
+            ps->chunk_begin = P.chunk_begin;
+            ps->chunk_end = P.chunk_end;
+            ps->n = P.n_paths;
+            ps->n_pixels = P.n_pixels;
+            ps->sample_base = 0;
+            ps->pixel_base = 0;
+            ps->n_pix = 0;
+            ps->k_samples = 0;
         } else {
-            uint32_t s0 = j * cfg->kmax;
-            done = s0 >= total;
-            sample_base = cfg->first_sample + s0;
-            pixel_base = 0;
-            n_pix = cfg->npix;
-            k = done ? 0 : min(cfg->kmax, total - s0);
+            uint32_t total = cfg->end_sample - cfg->first_sample;
+            uint32_t sample_base, pixel_base, n_pix, k;
+            bool done;
+            if (cfg->chunks > 1) {
+                uint32_t s = j / cfg->chunks, c = j % cfg->chunks;
+                done = s >= total;
+                sample_base = cfg->first_sample + s;
+                pixel_base = c * cfg->queue;
+                n_pix = min(cfg->queue, cfg->npix - pixel_base);
+                k = 1;
+            } else {
+                uint32_t s0 = j * cfg->kmax;
+                done = s0 >= total;
+                sample_base = cfg->first_sample + s0;
+                pixel_base = 0;
+                n_pix = cfg->npix;
+                k = done ? 0 : min(cfg->kmax, total - s0);
+            }
+            ps->done = done ? 1u : 0u;
+            ps->sample_base = sample_base;
+            ps->pixel_base = pixel_base;
+            ps->n_pix = done ? 0u : n_pix;
+            ps->k_samples = done ? 0u : k;
+            ps->n = done ? 0u : n_pix * k;
         }
-        ps->done = done ? 1u : 0u;
-        ps->sample_base = sample_base;
-        ps->pixel_base = pixel_base;
-        ps->n_pix = done ? 0u : n_pix;
-        ps->k_samples = done ? 0u : k;
-        ps->n = done ? 0u : n_pix * k;
         ps->pass_index = j + 1;
         n_sh = ps->n;
     }
@@ -344,6 +361,21 @@ __global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps) {
     if (threadIdx.x == 0) ps->countA[0] = n_sh;
 }
 
+// Last chunk c in [b, e) whose offset field (path_off or pix_off) is <= i.
+template <bool PIX>
+__device__ __forceinline__ uint32_t find_chunk(const ChunkDev* __restrict__ ch, uint32_t b, uint32_t e, uint32_t i) {
+    while (e - b > 1) {
+        const uint32_t m = (b + e) >> 1;
+        const uint32_t off = PIX ? ch[m].pix_off : ch[m].path_off;
+        if (off <= i) b = m;
+        else e = m;
+    }
+    return b;
+}
+
+__device__ __forceinline__ uint32_t batch_pixel(uint32_t field) { return field & ((1u << kBatchPixelBits) - 1u); }
+__device__ __forceinline__ uint32_t batch_image(uint32_t field) { return field >> kBatchPixelBits; }
+
 // Pixel-order index -> raster (x, y): bands of 8 rows, 8-column blocks inside a
 // band, so one wave of 64 consecutive paths covers an 8x8 tile.
 __device__ __forceinline__ void swizzle(uint32_t q, int W, int H, int& x, int& y) {
@@ -358,19 +390,41 @@ __device__ __forceinline__ void swizzle(uint32_t q, int W, int H, int& x, int& y
 }
 
 // ---- camera (cameras/perspective.cc:53-74, integrator.cc:84-86) ---------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cfg, RenderBufs rb) {
     const PassState* ps = rb.ps;
     const uint32_t n = ps->n;
     const uint32_t n_pix = ps->n_pix, pixel_base = ps->pixel_base, sample_base = ps->sample_base;
-    const int W = cfg->W, H = cfg->H;
-    const CameraDev cam = cfg->cam;
+    const uint32_t cb = ps->chunk_begin, ce = ps->chunk_end;
     const int kind = cfg->sampler;
-    const uint32_t seed = cfg->seed;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t s_local = i / n_pix, j = i - s_local * n_pix;
         int x, y;
-        swizzle(pixel_base + j, W, H, x, y);
-        uint32_t pixel = (uint32_t)(y * W + x), sample = sample_base + s_local;
+        uint32_t pixel, sample, flags, field;
+        CameraDev cam;
+        uint32_t seed;
+        if (BATCH) {
+            const ChunkDev ch = rb.chunks[find_chunk<false>(rb.chunks, cb, ce, i)];
+            const uint32_t local = i - ch.path_off;
+            const uint32_t s_local = local / ch.n_pix, j = local - s_local * ch.n_pix;
+            const ImageDev& im = rb.images[ch.img];
+            cam = im.cam;
+            seed = im.seed;
+            swizzle(ch.pixel_base + j, cam.W, cam.H, x, y);
+            pixel = (uint32_t)(y * cam.W + x);
+            sample = ch.sample_base + s_local;
+            flags = sample << 7;
+            field = pixel | (ch.img << kBatchPixelBits);
+        } else {
+            cam = cfg->cam;
+            seed = cfg->seed;
+            const uint32_t s_local = i / n_pix, j = i - s_local * n_pix;
+            swizzle(pixel_base + j, cfg->W, cfg->H, x, y);
+            pixel = (uint32_t)(y * cfg->W + x);
+            sample = sample_base + s_local;
+            flags = s_local << 7;
+            field = pixel;
+        }
+        const int W = cam.W, H = cam.H;
         float2 uf = sample_2d(kind, seed, pixel, sample, 0);
         float sx = 2.0f * ((float)x + uf.x) / (float)W - 1.0f;
         float sy = 1.0f - 2.0f * ((float)y + uf.y) / (float)H;
@@ -403,14 +457,14 @@ __global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cf
         float3 dw = normalize(mk(m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[4] * dir.x + m[5] * dir.y + m[6] * dir.z,
                                  m[8] * dir.x + m[9] * dir.y + m[10] * dir.z));
         rb.qa_o[0][i] = make_float4(ow.x, ow.y, ow.z, __uint_as_float(i));
-        rb.qa_d[0][i] = make_float4(dw.x, dw.y, dw.z, __uint_as_float(s_local << 7));
-        rb.qa_b[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(pixel));
+        rb.qa_d[0][i] = make_float4(dw.x, dw.y, dw.z, __uint_as_float(flags));
+        rb.qa_b[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(field));
         rb.path_L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
 // ---- extend: closest hit over the extend queue -------------------------------------------------
-template <int BLOCK, int S, bool COUNT>
+template <int BLOCK, int S, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, int depth) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
@@ -420,18 +474,22 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, in
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
     const float4* __restrict__ qo = rb.qa_o[depth & 1];
     const float4* __restrict__ qd = rb.qa_d[depth & 1];
+    const float4* __restrict__ qb = rb.qa_b[depth & 1];
     const WaveSplit ws = wave_split<BLOCK>();
     for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
         float3 o = mk(0, 0, 0), d = mk(0, 0, 1);
+        uint32_t img = 0;
         if (active) {
             o = xyz(qo[i]);
             d = xyz(qd[i]);
+            if (BATCH) img = batch_image(__float_as_uint(qb[i].w));
         }
+        const SceneDev& s = BATCH ? rb.scenes[img] : sc;
         float t, u, v;
         Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, false, COUNT>(sc, o, d, kINFTY, t, u, v, col, spill, active, cnt);
+        int32_t k = traverse<BLOCK, S, false, COUNT>(s, o, d, kINFTY, t, u, v, col, spill, active, cnt);
         if (active) rb.hit[i] = make_float4(t, u, v, __int_as_float(k));
         if (COUNT && active) {
             atomicAdd(&ps->count_nodes, (unsigned long long)cnt.nodes);
@@ -890,19 +948,18 @@ __device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
 }
 
 // ---- shade ---------------------------------------------------------------------------------------
-template <int BLOCK, bool GUIDED>
-__global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
+template <int BLOCK, bool GUIDED, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                  int depth) {
     PassState* ps = rb.ps;
     const uint32_t count = ps->countA[depth];
     if (count == 0) return;
     const int kind = cfgp->sampler;
-    const uint32_t seed = cfgp->seed;
+    const uint32_t cfg_seed = cfgp->seed;
     const int max_depth = cfgp->max_depth, rr_depth = cfgp->rr_depth;
     const uint32_t sample_base = ps->sample_base;
     const int cur = depth & 1, nxt = cur ^ 1;
     const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
-    const float nl = (float)sc.num_lights;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < count;
@@ -914,15 +971,21 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
         float s_tmax = 0.0f;
         float3 m_o = mk(0, 0, 0), m_d = mk(0, 0, 1), m_c = mk(0, 0, 0);
         int m_light = 0;
-        uint32_t path = 0, pixel = 0;
+        uint32_t path = 0, pixel = 0, pfield = 0, img = 0;
         float3 Ladd = mk(0, 0, 0);
         if (active) {
             float4 O = rb.qa_o[cur][i], D = rb.qa_d[cur][i], B = rb.qa_b[cur][i], H = rb.hit[i];
             path = __float_as_uint(O.w);
             const uint32_t flags = __float_as_uint(D.w);
             const bool specular = (flags >> 6) & 1u;
-            const uint32_t sample = sample_base + (flags >> 7);
-            pixel = __float_as_uint(B.w);
+            // batch entries carry the image in the pixel field and the absolute sample
+            pfield = __float_as_uint(B.w);
+            img = BATCH ? batch_image(pfield) : 0u;
+            pixel = BATCH ? batch_pixel(pfield) : pfield;
+            const SceneDev& sc = BATCH ? rb.scenes[img] : sc_arg;
+            const uint32_t seed = BATCH ? rb.images[img].seed : cfg_seed;
+            const float nl = (float)sc.num_lights;
+            const uint32_t sample = BATCH ? (flags >> 7) : sample_base + (flags >> 7);
             float3 ro = xyz(O), rd = xyz(D), beta = xyz(B);
             const int32_t k = __float_as_int(H.w);
             const bool emit = depth == 0 || specular;
@@ -1055,30 +1118,30 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc, const ConfigDev* _
             if (pushG) {
                 rb.qg[0][sg] = gq0;
                 rb.qg[1][sg] = gq1;
-                rb.qg[2][sg] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+                rb.qg[2][sg] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pfield));
                 rb.qg[3][sg] = gq3;
             }
         }
         if (pushA) {
             rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
             rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
-            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pfield));
         }
         if (pushS) {
             rb.qs_o[ss] = make_float4(s_o.x, s_o.y, s_o.z, __uint_as_float(path));
             rb.qs_d[ss] = make_float4(s_d.x, s_d.y, s_d.z, s_tmax);
-            rb.qs_c[ss] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
+            rb.qs_c[ss] = make_float4(s_c.x, s_c.y, s_c.z, __uint_as_float(img));
         }
         if (pushM) {
             rb.qm_o[sm] = make_float4(m_o.x, m_o.y, m_o.z, __uint_as_float(path));
             rb.qm_d[sm] = make_float4(m_d.x, m_d.y, m_d.z, __int_as_float(m_light));
-            rb.qm_c[sm] = make_float4(m_c.x, m_c.y, m_c.z, 0.0f);
+            rb.qm_c[sm] = make_float4(m_c.x, m_c.y, m_c.z, __uint_as_float(img));
         }
     }
 }
 
 // ---- shadow: NEE light-sample occlusion (visibility_tester.cc:21-24) ------------------------
-template <int BLOCK, int S>
+template <int BLOCK, int S, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, int depth) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
@@ -1090,17 +1153,19 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
     for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
-        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);
+        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0), C = make_float4(0, 0, 0, 0);
         if (active) {
             O = rb.qs_o[i];
             D = rb.qs_d[i];
+            if (BATCH) C = rb.qs_c[i];
         }
+        const SceneDev& s = BATCH ? rb.scenes[__float_as_uint(C.w)] : sc;
         float t, u, v;
         Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, true, false>(sc, xyz(O), xyz(D), D.w, t, u, v, col, spill, active, cnt);
+        int32_t k = traverse<BLOCK, S, true, false>(s, xyz(O), xyz(D), D.w, t, u, v, col, spill, active, cnt);
         if (active && k < 0) {
             uint32_t path = __float_as_uint(O.w);
-            float4 c = rb.qs_c[i];
+            float4 c = BATCH ? C : rb.qs_c[i];
             float4 Lv = rb.path_L[path];
             rb.path_L[path] = make_float4(Lv.x + c.x, Lv.y + c.y, Lv.z + c.z, Lv.w);
         }
@@ -1108,7 +1173,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
 }
 
 // ---- MIS: NEE BSDF-sample ray; counts if it reaches the chosen light (mis.cc:114-133) ----------
-template <int BLOCK, int S>
+template <int BLOCK, int S, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int depth) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
@@ -1120,26 +1185,28 @@ __global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int d
     for (uint32_t base = ws.first; base < count; base += ws.stride) {
         uint32_t i = base + __lane_id();
         bool active = i < count;
-        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0);
+        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0), C = make_float4(0, 0, 0, 0);
         if (active) {
             O = rb.qm_o[i];
             D = rb.qm_d[i];
+            if (BATCH) C = rb.qm_c[i];
         }
+        const SceneDev& s = BATCH ? rb.scenes[__float_as_uint(C.w)] : sc;
         float t, u, v;
         Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, false, false>(sc, xyz(O), xyz(D), kINFTY, t, u, v, col, spill, active, cnt);
+        int32_t k = traverse<BLOCK, S, false, false>(s, xyz(O), xyz(D), kINFTY, t, u, v, col, spill, active, cnt);
         if (active) {
             int li = __float_as_int(D.w);
             float3 Le = mk(0, 0, 0);
             if (k >= 0) {
-                if (li < sc.num_areas && sc.tri_ids[k].y == li && dot(xyz(sc.tri_ng[k]), neg(xyz(D))) > 0.0f)
-                    Le = xyz(sc.lights[li].L);
-            } else if (li >= sc.num_areas) {
-                Le = env_Le(sc.env, xyz(D));
+                if (li < s.num_areas && s.tri_ids[k].y == li && dot(xyz(s.tri_ng[k]), neg(xyz(D))) > 0.0f)
+                    Le = xyz(s.lights[li].L);
+            } else if (li >= s.num_areas) {
+                Le = env_Le(s.env, xyz(D));
             }
             if (!black(Le)) {
                 uint32_t path = __float_as_uint(O.w);
-                float4 c = rb.qm_c[i];
+                float4 c = BATCH ? C : rb.qm_c[i];
                 float4 Lv = rb.path_L[path];
                 rb.path_L[path] = make_float4(Lv.x + c.x * Le.x, Lv.y + c.y * Le.y, Lv.z + c.z * Le.z, Lv.w);
             }
@@ -1307,17 +1374,41 @@ __device__ __forceinline__ void kahan(float& s, float& c, float x) {
     c = (t - s) - y;
     s = t;
 }
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg, RenderBufs rb, int max_depth) {
     PassState* ps = rb.ps;
-    const uint32_t n_pix = ps->n_pix, k = ps->k_samples, pixel_base = ps->pixel_base;
-    const int W = cfg->W, H = cfg->H;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_pix; j += gridDim.x * blockDim.x) {
+    const uint32_t n_items = BATCH ? ps->n_pixels : ps->n_pix;
+    const uint32_t cb = ps->chunk_begin, ce = ps->chunk_end;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_items; t += gridDim.x * blockDim.x) {
+        // one film pixel: its k samples of this pass in sample order
+        uint32_t j, k, n_pix, pixel_base, path_base, film_off = 0;
+        int W, H;
+        if (BATCH) {
+            const ChunkDev ch = rb.chunks[find_chunk<true>(rb.chunks, cb, ce, t)];
+            const ImageDev& im = rb.images[ch.img];
+            j = t - ch.pix_off;
+            k = ch.k;
+            n_pix = ch.n_pix;
+            pixel_base = ch.pixel_base;
+            path_base = ch.path_off;
+            film_off = im.film_off;
+            W = im.cam.W;
+            H = im.cam.H;
+        } else {
+            j = t;
+            k = ps->k_samples;
+            n_pix = ps->n_pix;
+            pixel_base = ps->pixel_base;
+            path_base = 0;
+            W = cfg->W;
+            H = cfg->H;
+        }
         int x, y;
         swizzle(pixel_base + j, W, H, x, y);
-        uint32_t f = (uint32_t)(y * W + (W - 1 - x));
+        uint32_t f = film_off + (uint32_t)(y * W + (W - 1 - x));
         float4 s = rb.film_sum[f], c = rb.film_c[f], q = rb.film_sq[f], qc = rb.film_sqc[f];
         for (uint32_t si = 0; si < k; si++) {
-            float4 L = rb.path_L[si * n_pix + j];
+            float4 L = rb.path_L[path_base + si * n_pix + j];
             kahan(s.x, c.x, L.x);
             kahan(s.y, c.y, L.y);
             kahan(s.z, c.z, L.z);
@@ -1344,7 +1435,7 @@ __global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg,
         ps->total_extend += e;
         ps->total_shadow += sh;
         ps->total_mis += mi;
-        ps->total_samples += k;
+        ps->total_samples += ps->k_samples;
     }
 }
 
@@ -1446,9 +1537,9 @@ int trace_blocks_per_cu(int block) {
     int nb = 0;
     hipError_t e;
     switch (block) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<64, kStackSlots, false>, 64, 0); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<256, kStackSlots, false>, 256, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots, false>, 128, 0); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<64, kStackSlots, false, false>, 64, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<256, kStackSlots, false, false>, 256, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<128, kStackSlots, false, false>, 128, 0); break;
     }
     if (e != hipSuccess || nb < 1) return 2;
     return nb;
@@ -1494,30 +1585,31 @@ int launch_relmse_pixels(const RenderBufs& b, int64_t n, const int32_t* pixels, 
     return SPK_OK;
 }
 
-template <int B>
+template <int B, bool BATCH>
 static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
                          hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
-    hipLaunchKernelGGL(k_pass_init, dim3(1), dim3(256), 0, st, b.cfg, b.ps);
-    hipLaunchKernelGGL(k_camera, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
+    hipLaunchKernelGGL(k_pass_init<BATCH>, dim3(1), dim3(256), 0, st, b.cfg, b.ps, b.passes, b.num_passes);
+    hipLaunchKernelGGL(k_camera<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
     for (int d = 0; d <= cfg.max_depth; d++) {
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d], st));
         if (count)
-            hipLaunchKernelGGL((k_extend<B, kStackSlots, true>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_extend<B, kStackSlots, true, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         else
-            hipLaunchKernelGGL((k_extend<B, kStackSlots, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_extend<B, kStackSlots, false, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
-        if (guide)
-            hipLaunchKernelGGL((k_shade<B, true>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+        if (!BATCH && guide)
+            hipLaunchKernelGGL((k_shade<B, true, false>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
         else
-            hipLaunchKernelGGL((k_shade<B, false>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+            hipLaunchKernelGGL((k_shade<B, false, BATCH>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
         if (d < cfg.max_depth) {
-            hipLaunchKernelGGL((k_shadow<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
-            hipLaunchKernelGGL((k_mis<B, kStackSlots>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_mis<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
             // after the NEE kernels, so a training record sees this vertex's direct light
-            if (guide) hipLaunchKernelGGL(k_guide, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
+            if (!BATCH && guide)
+                hipLaunchKernelGGL(k_guide, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
         }
     }
-    hipLaunchKernelGGL(k_film, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
+    hipLaunchKernelGGL(k_film<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
 }
@@ -1525,9 +1617,19 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
                 hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
     switch (ls.block) {
-    case 64: return launch_pass_b<64>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
-    case 256: return launch_pass_b<256>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
-    default: return launch_pass_b<128>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    case 64: return launch_pass_b<64, false>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    case 256: return launch_pass_b<256, false>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    default: return launch_pass_b<128, false>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
+    }
+}
+
+int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st) {
+    SceneDev none;
+    memset(&none, 0, sizeof(none));
+    switch (ls.block) {
+    case 64: return launch_pass_b<64, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
+    case 256: return launch_pass_b<256, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
+    default: return launch_pass_b<128, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
     }
 }
 

@@ -153,6 +153,70 @@ class GpuRender:
             pass
 
 
+class Batch:
+    """Many images (scene, camera, spp, seed each) rendered through shared wavefront passes."""
+
+    def __init__(self, items, max_depth=16, rr_depth=3, sampler=_abi.SPK_SAMPLER_SOBOL, queue_size=0, block_size=0,
+                 graph=1):
+        """items: list of (GpuScene, Camera, spp, seed)."""
+        self.lib = _abi.load()
+        self.items = list(items)
+        arr = (_abi.SpkBatchImage * len(self.items))()
+        for a, (gs, cam, spp, seed) in zip(arr, self.items):
+            a.scene = gs.handle
+            a.camera = cam.desc()
+            a.spp = int(spp)
+            a.seed = int(seed)
+        self._arr = arr
+        d = _abi.SpkBatchDesc()
+        d.num_images = len(self.items)
+        d.images = arr
+        d.max_depth = max_depth
+        d.rr_depth = rr_depth
+        d.sampler = sampler
+        d.graph = graph
+        o = _abi.SpkRenderOpts()
+        o.queue_size = queue_size
+        o.block_size = block_size
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.spk_batch_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)), "spk_batch_create")
+        self.handle = h
+
+    def run(self, sync=True):
+        _abi.check(self.lib.spk_batch_run(self.handle, 1 if sync else 0), "spk_batch_run")
+
+    def film(self, image):
+        cam = self.items[image][1]
+        w, h = cam.width, cam.height
+        s = np.zeros((h, w, 3))
+        q = np.zeros((h, w, 3))
+        c = np.zeros((h, w))
+        _abi.check(self.lib.spk_batch_film(self.handle, image, s.ctypes.data_as(_abi.c_d_p), q.ctypes.data_as(_abi.c_d_p),
+                                           c.ctypes.data_as(_abi.c_d_p)), "spk_batch_film")
+        return s, q, c
+
+    def info(self):
+        st = _abi.SpkBatchInfo()
+        _abi.check(self.lib.spk_batch_info_get(self.handle, ctypes.byref(st)), "spk_batch_info_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def stats(self):
+        st = _abi.SpkRenderStats()
+        _abi.check(self.lib.spk_batch_stats_get(self.handle, ctypes.byref(st)), "spk_batch_stats_get")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.spk_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def guide_opts(**kw):
     o = _abi.SpkGuideOpts()
     _abi.check(_abi.load().spk_guide_opts_default(ctypes.byref(o)), "spk_guide_opts_default")
