@@ -10,7 +10,8 @@ Printed (one JSON line): total traced rays per second (camera + continuation +
 shadow + MIS rays, all traversals), the closest-hit traversal kernel against the
 HBM roofline, the relMSE of the rendered C1 image against its 65536-spp float64 CPU
 reference, the wall time to relMSE 1e-3 of plain and guided tracing on the
-occluded-glossy scene (C3, online-trained proposal included in the guided time), and
+occluded-glossy scene (C3, online-trained proposal included in the guided time), the
+64-image batch (C2) throughput, and
 spica's CPU path (the FP64 C restatement, the reference's estimator and BVH) timed
 on this host's cores on a bounded sample.
 """
@@ -34,13 +35,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(profile_dir):
-    """HBM bytes per closest-hit launch from committed rocprofv3 PMC summaries, or None."""
+def read_traffic(profile_dir, rays_per_launch):
+    """HBM bytes per closest-hit launch: the committed rocprofv3 PMC figure per ray times
+    this run's rays per launch (the profile states its own queue/block), or None."""
     path = os.path.join(profile_dir, "extend_c1_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    if "bytes_per_ray" not in d:
+        return None
+    return round(d["bytes_per_ray"] * rays_per_launch)
 
 
 def cpu_baseline(w, threads, budget_s):
@@ -133,6 +138,7 @@ def main():
     ap.add_argument("--queue", type=int, default=0, help="paths in flight (0 = autotune)")
     ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -243,6 +249,37 @@ def main():
                    guide=dict(guide_cfg, network="one-blob(7x32) -> 64x4 ReLU -> 8 vMF lobes, bf16 MFMA, fp32 accum"))
         gs3.close()
 
+    batch = None
+    if not args.no_batch:
+        log("bench: C2 batch: generating 64 scenes")
+        t0 = time.perf_counter()
+        wb = configs.c2_batch()
+        scenes = [render.GpuScene(sc) for sc, _, _, _ in wb.items]
+        tris = sum(g.stats()["num_tris"] for g in scenes)
+        setup_s = time.perf_counter() - t0
+        items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
+        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
+                         block_size=args.block, graph=1)
+        st0 = b.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(sync=True)
+        torch.cuda.synchronize()
+        bdt = time.perf_counter() - t0
+        st1 = b.stats()
+        brays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+        info = b.info()
+        batch = dict(workload="C2: 64 images, image i = C1 generator seed i with U{16..2048} spheres, W,H ~ U{128..1024}, "
+                              "spp ~ U{16..512}, maxDepth 16, Sobol seed i; one per-pass hipGraph replayed over shared passes",
+                     images=len(items), triangles=int(tris), paths=int(info["paths"]), passes=int(info["passes"]),
+                     seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
+                     mpaths_per_s=round(info["paths"] / bdt / 1e6, 3), setup_s=round(setup_s, 1),
+                     queue=int(info["queue"]), block=args.block)
+        log(f"bench: batch {batch}")
+        b.close()
+        for g in scenes:
+            g.close()
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_threads} threads for ~{args.cpu_seconds:.0f} s")
@@ -273,7 +310,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(os.path.join(ROOT, "profiles")),
+            "traffic": read_traffic(os.path.join(ROOT, "profiles"), ext_rays / max(1, ext_launches)),
             "b_ray": round(b_ray, 1),
             "v_node": round(v_node, 3),
             "v_tri": round(v_tri, 3),
@@ -282,6 +319,7 @@ def main():
         },
         "relmse": c1_relmse,
         "time_to_relmse": ttr,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

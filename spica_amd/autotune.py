@@ -2,14 +2,15 @@
 
 Deep bounces are latency bound (few long rays per launch), so larger queues amortise
 them over more samples per pass, at the cost of queue memory (~224 B per path).
-The tuner times a few passes per candidate, keeps the fastest by traced rays per
+The tuner times two full passes per candidate (a probe shorter than a pass would
+never fill a large queue and understate it), keeps the fastest by traced rays per
 second and caches the choice per (scene size, film, depth, device).
 """
 import json
 import os
 import time
 
-DEFAULT_QUEUES = (1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25)
+DEFAULT_QUEUES = (1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
 DEFAULT_BLOCKS = (64, 128, 256)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "spica_amd", "autotune.json")
 
@@ -38,7 +39,8 @@ def _store(db):
         pass
 
 
-def tune(gscene, camera, desc, queues=DEFAULT_QUEUES, blocks=DEFAULT_BLOCKS, spp=8, use_cache=True, log=None):
+def tune(gscene, camera, desc, queues=DEFAULT_QUEUES, blocks=DEFAULT_BLOCKS, spp=8, use_cache=True, log=None,
+         max_spp=64):
     """Returns (queue_size, block_size, table) for this scene/film/depth."""
     import torch
 
@@ -53,22 +55,24 @@ def tune(gscene, camera, desc, queues=DEFAULT_QUEUES, blocks=DEFAULT_BLOCKS, spp
     table = []
     best = None
     for q in queues:
-        if q // npix > 2 * spp and q > queues[0]:
-            continue  # more samples per pass than the probe runs: no information
+        per_pass = max(1, q // npix)
+        n = max(spp, 2 * per_pass)   # two full passes
+        if n > max_spp and q > queues[0]:
+            continue
         for b in blocks:
             gr = render.GpuRender(gscene, camera, desc, queue_size=q, block_size=b)
-            gr.run(max(1, min(spp, q // npix)))
+            gr.run(per_pass)
             st0 = gr.stats()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            gr.run(spp)
+            gr.run(n)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             st1 = gr.stats()
             gr.close()
             rays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
             rate = rays / dt / 1e6
-            table.append(dict(queue=q, block=b, mrays_per_s=round(rate, 1), ms_per_spp=round(dt * 1e3 / spp, 3)))
+            table.append(dict(queue=q, block=b, mrays_per_s=round(rate, 1), ms_per_spp=round(dt * 1e3 / n, 3)))
             if log:
                 log(f"autotune: queue 2^{q.bit_length() - 1} block {b}: {rate:.1f} Mrays/s")
             if best is None or rate > best[0]:

@@ -319,11 +319,29 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
         spk_scene_destroy(s);
         check(rc, "spk_render_create");
     }
+    // "guiding" (GPU key, default 0): the learned proposal, trained online during the first
+    // "guideTrainSpp" samples per pixel and MIS-combined with the BSDF.
+    spk_guide* g = nullptr;
+    if (params.getInt("guiding", 0)) {
+        spk_guide_opts go;
+        spk_guide_opts_default(&go);
+        go.train_spp = params.getInt("guideTrainSpp", go.train_spp);
+        go.seed = (uint32_t)params.getInt("guideSeed", (int)go.seed);
+        rc = spk_guide_create(params.getInt("device", 0), &go, &g);
+        if (rc == SPK_OK) rc = spk_render_set_guide(r, g);
+        if (rc != SPK_OK) {
+            spk_render_destroy(r);
+            if (g) spk_guide_destroy(g);
+            spk_scene_destroy(s);
+            check(rc, "guiding");
+        }
+    }
     rc = spk_render_run(r, spp, 1);
     std::vector<double> sum((size_t)cd.width * cd.height * 3), cnt((size_t)cd.width * cd.height);
     if (rc == SPK_OK) rc = spk_render_film(r, sum.data(), nullptr, cnt.data());
     if (rc == SPK_OK) rc = spk_render_stats_get(r, &stats_);
     spk_render_destroy(r);
+    if (g) spk_guide_destroy(g);
     spk_scene_destroy(s);
     check(rc, "render");
     // Film::save's division (film.cc:23-40)

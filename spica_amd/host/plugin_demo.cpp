@@ -40,7 +40,7 @@ std::shared_ptr<SurfaceMaterial> material_from(const spk_material& m) {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s scene.bin out.f64 [plugin.so]\n", argv[0]);
+        fprintf(stderr, "usage: %s scene.bin out.f64 [plugin.so] [guiding]\n", argv[0]);
         return 2;
     }
     std::string plugin = argc > 3 ? argv[3] : "libspica_path.so";
@@ -97,6 +97,10 @@ int main(int argc, char** argv) {
     params.add("maxDepth", (int)max_depth);
     if (sampler == SPK_SAMPLER_SOBOL) params.add("sampler", std::shared_ptr<CObject>(std::make_shared<Sobol>(seed)));
     else params.add("sampler", std::shared_ptr<CObject>(std::make_shared<Independent>(seed)));
+    if (argc > 4 && std::string(argv[4]) == "guiding") {
+        params.add("guiding", 1);
+        params.add("guideTrainSpp", 16);
+    }
     std::shared_ptr<Integrator> integrator(dynamic_cast<Integrator*>(create(params)));
     printf("plugin: %s\n", describe());
 
@@ -130,8 +134,9 @@ int main(int argc, char** argv) {
     const auto* pi = dynamic_cast<PathIntegrator*>(integrator.get());
     if (pi) {
         const spk_render_stats& st = pi->lastStats();
-        printf("paths %lld rays extend %lld shadow %lld mis %lld\n", (long long)st.paths, (long long)st.rays_extend,
-               (long long)st.rays_shadow, (long long)st.rays_mis);
+        printf("paths %lld rays extend %lld shadow %lld mis %lld guided %lld\n", (long long)st.paths,
+               (long long)st.rays_extend, (long long)st.rays_shadow, (long long)st.rays_mis,
+               (long long)st.guided_vertices);
     }
     FILE* o = fopen(argv[2], "wb");
     if (!o) return 5;

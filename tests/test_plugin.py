@@ -52,3 +52,25 @@ def test_plugin_render_equals_c_abi_render(gpu, tmp_path):
     want = s / (c[..., None] + 1e-12)
     assert np.array_equal(img, want)
     assert "rays extend" in r.stdout
+
+
+@pytest.mark.gpu
+def test_plugin_guiding_parameter(gpu, tmp_path):
+    """"guiding" = 1 attaches the learned proposal inside the plugin's render: guided
+    vertices are reported and the image agrees with the plain render in the mean."""
+    w = configs.cornell_box(width=64, height=48)
+    sf = tmp_path / "s.bin"
+    w.scene.save_bin(sf, w.camera, 256, 8, 0, 0)
+    out = tmp_path / "img.f64"
+    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN, "guiding"], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    guided = int(r.stdout.split("guided")[1].split()[0])
+    assert guided > 0
+    img = np.fromfile(out).reshape(48, 64, 3)
+    gs = gpu.GpuScene(w.scene)
+    gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=256, seed=99))
+    gr.run(256)
+    s, q, c = gr.film()
+    plain = s / (c[..., None] + 1e-12)
+    assert abs(img.mean() / plain.mean() - 1) < 0.01
