@@ -4,6 +4,7 @@ estimator, independent sampler with a seed no GPU run uses).
     python tests/golden/make_reference.py c0            # 256x256 full frame, 65536 spp
     python tests/golden/make_reference.py c3 [nx]        # 1280x720, 160x90 stratified pixels, 65536 spp
     python tests/golden/make_reference.py c1             # 1920x1080, 160x90 stratified pixels, 65536 spp
+    python tests/golden/make_reference.py c4             # view 0 of the heavy scene, 80x45 stratified pixels, 65536 spp
 
 Writes tests/golden/<name>_ref.npz with the film indices, per-pixel mean and the
 standard error of the mean.  CPU-only; run in the build container.
@@ -61,3 +62,6 @@ if __name__ == "__main__":
     elif which == "c1":
         wl = configs.c1_spheres()
         render("c1", wl, subset_pixels(wl.camera.width, wl.camera.height, 160, 90), 65536, 1024, threads)
+    elif which == "c4":
+        wl = configs.c4_heavy()[0]
+        render("c4", wl, subset_pixels(wl.camera.width, wl.camera.height, 80, 45), 65536, 1024, threads)
