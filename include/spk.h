@@ -190,6 +190,10 @@ typedef struct spk_guide_stats {
 } spk_guide_stats;
 int spk_guide_stats_get(spk_guide* guide, spk_guide_stats* out);
 
+/* Samples per pixel a render trains the guide for after spk_render_set_guide (the
+ * train_spp option); 0 freezes the network, e.g. to reuse it for other views of a scene. */
+int spk_guide_set_train_spp(spk_guide* guide, int32_t train_spp);
+
 /* Attach a guide to a render (NULL detaches: plain BSDF sampling again).  The render
  * trains it during its first train_spp samples per pixel after attaching.  One guide
  * serves one render at a time. */
@@ -203,8 +207,8 @@ int spk_render_set_guide(spk_render* render, spk_guide* guide);
  * the film spk_render_run gives for it alone, bit for bit (samples are keyed by pixel,
  * sample and dimension, and every pixel accumulates in sample order).  Replaces running
  * spica's PathIntegrator::render once per scene file (sources/integrators/path/path.cc,
- * sources/core/integrator.cc:62-107).  Limits: W * H <= 2^20 per image, up to 4096
- * images, no environment-map or guide per batch image. */
+ * sources/core/integrator.cc:62-107).  Limits: W * H <= 2^22 per image, up to 1024
+ * images, no guide (a guide serves single renders; views of one scene can share it). */
 typedef struct spk_batch spk_batch;
 
 typedef struct spk_batch_image {

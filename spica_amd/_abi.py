@@ -219,6 +219,7 @@ SIGNATURES = {
     "spk_guide_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, c_f_p, c_d_p]),
     "spk_guide_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f_p, c_f_p, c_f_p, ctypes.c_int32]),
     "spk_guide_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkGuideStats)]),
+    "spk_guide_set_train_spp": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "spk_render_set_guide": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "spk_batch_create": (ctypes.c_int, [ctypes.POINTER(SpkBatchDesc), ctypes.POINTER(SpkRenderOpts),
                                         ctypes.POINTER(ctypes.c_void_p)]),

@@ -12,8 +12,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed FP32 arithmetic is disabled for all device code: hipcc would otherwise
-# emit it from plain float code.
-FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17", "-Wall",
+# emit it from plain float code.  Multiply-adds are fused only within one source
+# expression (-ffp-contract=on), so a function's rounding does not depend on the kernel
+# it is inlined into (batched and single renders then agree bit for bit).
+FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17", "-Wall", "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

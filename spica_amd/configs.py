@@ -11,6 +11,10 @@ C1  Sphere field: seed-1 generator, 512 spheres of 640 triangles (UV sphere 20 x
 C2  Batch: 64 images; image i draws its scene from the C1 generator with seed i and
     U{16..2048} spheres (10k-1.3M triangles), a W x H film with W, H ~ U{128..1024} and
     U{16..512} samples per pixel; maxDepth 16, Sobol seed i.
+C4  Heavy combined: seed 11; 6000 spheres of the C1 generator + two noise-displaced
+    height-field grids of 1,002,528 triangles each (ground and back wall), 4 quad lights
+    and a 1024x512 synthetic HDR sky (gradient 0.5-2.0, sun disc of radiance 1e4),
+    8 views at 1920x1080, 2048 spp, maxDepth 32.
 C3  Occluded glossy: the same generator with seed 7, 60% GGX alpha U[0.1,0.3], one
     quad light inside a box open on its top that lights the scene via the ceiling;
     1280x720, 1024 spp, maxDepth 16.
@@ -92,8 +96,8 @@ C1_MIX = [(0.4, ("lambert",)), (0.2, ("mirror",)), (0.2, ("dielectric",)), (0.2,
 C3_MIX = [(0.6, ("ggx", 0.1, 0.3)), (0.25, ("lambert",)), (0.1, ("mirror",)), (0.05, ("dielectric",))]
 
 
-def sphere_field(seed=1, n_spheres=512, mix=None, lights="quads"):
-    """The sphere-field generator (C1; C2 and C3 reuse it)."""
+def sphere_field(seed=1, n_spheres=512, mix=None, lights="quads", ground=True):
+    """The sphere-field generator (C1; C2, C3 and C4 reuse it)."""
     rng = np.random.default_rng(seed)
     mix = C1_MIX if mix is None else mix
     s = Scene()
@@ -104,7 +108,8 @@ def sphere_field(seed=1, n_spheres=512, mix=None, lights="quads"):
         m = _sphere_materials(rng, mix)
         s.add_triangles(unit * radii[i] + centers[i], m)
     g = -5.5
-    s.add_triangles(quad((-10, g, -10), (-10, g, 10), (10, g, 10), (10, g, -10)), diffuse((0.5, 0.5, 0.5)))
+    if ground:
+        s.add_triangles(quad((-10, g, -10), (-10, g, 10), (10, g, 10), (10, g, -10)), diffuse((0.5, 0.5, 0.5)))
     black = s.add_material(diffuse((0.0, 0.0, 0.0)))
     if lights == "quads":
         for cx in (-4.0, 4.0):
@@ -166,3 +171,83 @@ def c3_occluded_glossy(width=1280, height=720, spp=1024, max_depth=16, seed=7, n
                     radiance=(60.0, 60.0, 60.0))
     cam = Camera(width, height, look_at((0.0, 2.0, -18.0), (0.0, -1.0, 0.0), (0.0, 1.0, 0.0)), 40.0)
     return Workload("C3 occluded glossy", s, cam, spp, max_depth, _abi.SPK_SAMPLER_SOBOL, seed)
+
+
+def _noise(rng, u, v, octaves=5):
+    """Smooth deterministic noise on [0,1]^2: a sum of randomly oriented sinusoids."""
+    h = np.zeros_like(u)
+    amp, freq = 1.0, 2.0
+    for _ in range(octaves):
+        for _ in range(3):
+            th = rng.uniform(0, 2 * math.pi)
+            ph = rng.uniform(0, 2 * math.pi)
+            h += amp * np.sin(freq * 2 * math.pi * (u * math.cos(th) + v * math.sin(th)) + ph)
+        amp *= 0.5
+        freq *= 2.1
+    return h / 3.0
+
+
+def noise_grid(rng, n, origin, eu, ev, normal_dir, amplitude):
+    """n x n quads (2 n^2 triangles) over origin + s eu + t ev, displaced along normal_dir."""
+    t = np.linspace(0.0, 1.0, n + 1)
+    U, V = np.meshgrid(t, t, indexing="ij")
+    H = amplitude * _noise(rng, U, V)
+    P = (np.asarray(origin)[None, None, :] + U[..., None] * np.asarray(eu)[None, None, :]
+         + V[..., None] * np.asarray(ev)[None, None, :] + H[..., None] * np.asarray(normal_dir)[None, None, :])
+    a, b, c, d = P[:-1, :-1], P[1:, :-1], P[1:, 1:], P[:-1, 1:]
+    t1 = np.stack([a, b, c], axis=2).reshape(-1, 3, 3)
+    t2 = np.stack([a, c, d], axis=2).reshape(-1, 3, 3)
+    return np.concatenate([t1, t2]).astype(np.float32)
+
+
+def sky_envmap(w=1024, h=512, sun_theta_deg=40.0, sun_phi_deg=60.0, sun_radius_deg=0.75, sun=1.0e4):
+    """Lat-long z-up sky (u = phi / 2pi, v = theta / pi): luminance 2.0 at the horizon to 0.5 at
+    the zenith (blue-tinted), 0.5 below the horizon, and a sun disc of radiance `sun`."""
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    theta = v * math.pi
+    up = np.clip(np.cos(theta), 0.0, 1.0)
+    lum = np.where(theta <= math.pi / 2, 2.0 - 1.5 * up, 0.5)
+    img = np.stack([0.8 * lum, 0.9 * lum, 1.1 * lum], axis=-1)[:, None, :].repeat(w, axis=1)
+    st, sp = math.radians(sun_theta_deg), math.radians(sun_phi_deg)
+    sdir = np.array([math.sin(st) * math.cos(sp), math.sin(st) * math.sin(sp), math.cos(st)])
+    TH, PH = np.meshgrid(theta, u * 2 * math.pi, indexing="ij")
+    d = np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], axis=-1)
+    img[(d @ sdir) >= math.cos(math.radians(sun_radius_deg))] = sun
+    return img.astype(np.float32)
+
+
+C4_VIEWS = 8
+
+
+def c4_scene(seed=11, n_spheres=6000, grid_n=708):
+    s, black = sphere_field(seed, n_spheres, lights=None, ground=False)
+    rng = np.random.default_rng(seed + 1000)
+    s.add_triangles(noise_grid(rng, grid_n, (-12.0, -5.5, -12.0), (24.0, 0.0, 0.0), (0.0, 0.0, 24.0), (0.0, 1.0, 0.0), 0.4),
+                    diffuse((0.5, 0.5, 0.5)))
+    s.add_triangles(noise_grid(rng, grid_n, (-12.0, -5.5, 9.0), (0.0, 16.0, 0.0), (24.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0.6),
+                    diffuse((0.6, 0.55, 0.5)))
+    for cx, cz in ((-4.0, -4.0), (4.0, -4.0), (-4.0, 4.0), (4.0, 4.0)):
+        y, hh = 9.0, 0.75
+        s.add_triangles(quad((cx - hh, y, cz - hh), (cx + hh, y, cz - hh), (cx + hh, y, cz + hh), (cx - hh, y, cz + hh)),
+                        black, radiance=(12.0, 12.0, 12.0))
+    # z-up sky onto the y-up world
+    s.set_envmap(sky_envmap(), to_world=np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], float),
+                 radius=60.0)
+    return s
+
+
+def c4_cameras(width=1920, height=1080, views=C4_VIEWS):
+    cams = []
+    for i in range(views):
+        a = math.radians(-90.0 + (i - (views - 1) / 2.0) * 12.0)
+        eye = (18.0 * math.cos(a), 3.0 + 0.5 * (i % 3), 18.0 * math.sin(a))
+        cams.append(Camera(width, height, look_at(eye, (0.0, -1.0, 0.0), (0.0, 1.0, 0.0)), 40.0))
+    return cams
+
+
+def c4_heavy(width=1920, height=1080, spp=2048, max_depth=32, seed=11, n_spheres=6000, grid_n=708, views=C4_VIEWS):
+    """The heavy combined workload: one scene, `views` cameras (Sobol seed = 11 + view)."""
+    s = c4_scene(seed, n_spheres, grid_n)
+    return [Workload(f"C4 heavy view {i}", s, cam, spp, max_depth, _abi.SPK_SAMPLER_SOBOL, seed + i)
+            for i, cam in enumerate(c4_cameras(width, height, views))]

@@ -1191,7 +1191,8 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
             return SPK_ERR_INVALID;
         }
         if (c.width <= 0 || c.height <= 0 || (int64_t)c.width * c.height > (int64_t(1) << kBatchPixelBits)) {
-            set_error("spk_batch_create: image %d film %dx%d outside 1 .. 2^20 pixels", i, c.width, c.height);
+            set_error("spk_batch_create: image %d film %dx%d outside 1 .. 2^%u pixels", i, c.width, c.height,
+                      kBatchPixelBits);
             return SPK_ERR_INVALID;
         }
         if (c.to_world[12] != 0.0 || c.to_world[13] != 0.0 || c.to_world[14] != 0.0 || c.to_world[15] != 1.0) {
@@ -1200,10 +1201,6 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
         }
         if (im.spp <= 0 || im.spp >= (1 << 25)) {
             set_error("spk_batch_create: image %d spp %d outside [1, 2^25)", i, im.spp);
-            return SPK_ERR_INVALID;
-        }
-        if (im.scene->dev.env.present) {
-            set_error("spk_batch_create: image %d scene has an environment map (not batched)", i);
             return SPK_ERR_INVALID;
         }
         ImageDev id;

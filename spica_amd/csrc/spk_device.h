@@ -80,8 +80,8 @@ static constexpr int kMaxDepthCap = 64;
 
 // ---- batched rendering (many images, each its own scene, in shared passes) ----
 // Queue entries of a batch carry pixel | image << kBatchPixelBits in their pixel field
-// and the absolute sample index in flag bits 7..31.
-static constexpr uint32_t kBatchPixelBits = 20;
+// and the absolute sample index in flag bits 7..31 (images up to 4 M pixels, up to 1024 images).
+static constexpr uint32_t kBatchPixelBits = 22;
 static constexpr uint32_t kBatchMaxImages = 1u << (32 - kBatchPixelBits);
 
 struct ImageDev {

@@ -332,6 +332,13 @@ int spk_guide_train(spk_guide* g, int64_t n, const float* x, const float* dirs, 
     return direct_steps(g, n, x, dirs, weight, steps, nullptr, nullptr);
 }
 
+int spk_guide_set_train_spp(spk_guide* g, int32_t train_spp) {
+    clear_error();
+    if (!g || train_spp < 0) return SPK_ERR_INVALID;
+    g->opts.train_spp = train_spp;
+    return SPK_OK;
+}
+
 int spk_guide_stats_get(spk_guide* g, spk_guide_stats* out) {
     clear_error();
     if (!g || !out) return SPK_ERR_INVALID;

@@ -289,6 +289,10 @@ class Guide:
         _abi.check(self.lib.spk_guide_train(self.handle, n, x.ctypes.data_as(_abi.c_f_p), dirs.ctypes.data_as(_abi.c_f_p),
                                             weight.ctypes.data_as(_abi.c_f_p), int(steps)), "spk_guide_train")
 
+    def set_train_spp(self, n):
+        """Training samples per pixel of the next renders it is attached to (0 = frozen)."""
+        _abi.check(self.lib.spk_guide_set_train_spp(self.handle, int(n)), "spk_guide_set_train_spp")
+
     def stats(self):
         st = _abi.SpkGuideStats()
         _abi.check(self.lib.spk_guide_stats_get(self.handle, ctypes.byref(st)), "spk_guide_stats_get")

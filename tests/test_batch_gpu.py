@@ -10,7 +10,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _small_batch():
-    return configs.c2_batch(n_images=5, draw_seed=7, spheres=(4, 40), res=(24, 96), spp=(3, 40))
+    wl = configs.c2_batch(n_images=5, draw_seed=7, spheres=(4, 40), res=(24, 96), spp=(3, 40))
+    # one image lit by an environment map as well (each image's scene carries its own)
+    wl.items[1][0].set_envmap(configs.sky_envmap(64, 32, sun=200.0),
+                              to_world=np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], float),
+                              radius=40.0)
+    return wl
 
 
 @pytest.fixture(scope="module")
