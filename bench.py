@@ -11,7 +11,8 @@ shadow + MIS rays, all traversals), the closest-hit traversal kernel against the
 HBM roofline, the relMSE of the rendered C1 image against its 65536-spp float64 CPU
 reference, the wall time to relMSE 1e-3 of plain and guided tracing on the
 occluded-glossy scene (C3, online-trained proposal included in the guided time), the
-64-image batch (C2) throughput, and
+64-image batch (C2) throughput, the heavy scene (C4: 8-view throughput, plain and guided
+relMSE of view 0), and
 spica's CPU path (the FP64 C restatement, the reference's estimator and BVH) timed
 on this host's cores on a bounded sample.
 """
@@ -139,6 +140,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
+    ap.add_argument("--no-heavy", action="store_true", help="skip the heavy-scene (C4) leg")
+    ap.add_argument("--heavy-spp", type=int, default=256, help="samples per pixel of the C4 relMSE renders")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -280,6 +283,56 @@ def main():
         for g in scenes:
             g.close()
 
+    heavy = None
+    if not args.no_heavy:
+        log("bench: C4 heavy scene")
+        t0 = time.perf_counter()
+        views = configs.c4_heavy()
+        gs4 = render.GpuScene(views[0].scene)
+        st4 = gs4.stats()
+        setup_s = time.perf_counter() - t0
+        vspp = 16
+        items = [(gs4, v.camera, vspp, v.seed) for v in views]
+        b = render.Batch(items, max_depth=views[0].max_depth, sampler=views[0].sampler, queue_size=args.queue,
+                         block_size=args.block, graph=1)
+        b.run()
+        st0 = b.stats()
+        t0 = time.perf_counter()
+        b.run(sync=True)
+        bdt = time.perf_counter() - t0
+        st1 = b.stats()
+        brays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+        bpaths = st1["paths"] - st0["paths"]
+        b.close()
+        heavy = dict(workload="C4: seed 11, 6000 spheres + 2 noise-displaced 1M-triangle grids, 1024x512 HDR sky "
+                              "(sun 1e4, gradient 0.5-2.0), 4 quad lights; 8 views 1920x1080, maxDepth 32, Sobol",
+                     triangles=int(st4["num_tris"]), bvh_build_ms=round(st4["build_ms"], 1), setup_s=round(setup_s, 1),
+                     batch=dict(views=len(items), spp=vspp, seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
+                                mpaths_per_s=round(bpaths / bdt / 1e6, 3), rays_per_path=round(brays / bpaths, 3)))
+        c4_ref = load_reference("c4")
+        if c4_ref is not None:
+            pix, refm, rspp = c4_ref
+            res = {}
+            for name, guided in (("plain", False), ("guided", True)):
+                v = views[0]
+                gr = render.GpuRender(gs4, v.camera, v.render_desc(spp=args.heavy_spp), queue_size=args.queue,
+                                      block_size=args.block)
+                g = None
+                if guided:
+                    g = render.Guide(train_spp=64, train_pass_spp=4, steps_per_pass=2)
+                    gr.set_guide(g)
+                t0 = time.perf_counter()
+                gr.run(args.heavy_spp, sync=True)
+                dt = time.perf_counter() - t0
+                res[name] = dict(spp=args.heavy_spp, seconds=round(dt, 3),
+                                 relmse=float(f"{gr.relmse_pixels(pix, refm):.5g}"))
+                gr.close()
+                if g is not None:
+                    g.close()
+            heavy["view0"] = dict(res, reference=f"{rspp}-spp float64 CPU render, {len(pix)} stratified pixels of view 0")
+        log(f"bench: heavy {heavy}")
+        gs4.close()
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_threads} threads for ~{args.cpu_seconds:.0f} s")
@@ -320,6 +373,7 @@ def main():
         "relmse": c1_relmse,
         "time_to_relmse": ttr,
         "batch": batch,
+        "heavy": heavy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -49,7 +49,8 @@ typedef struct spk_material {
 typedef struct spk_scene_desc {
     int64_t num_tris;
     const float* verts;          /* [num_tris][3][3] world-space vertex positions */
-    const float* normals;        /* [num_tris][3][3] per-vertex normals or NULL (face normals) */
+    const float* normals;        /* [num_tris][3][3] per-vertex normals or NULL (face normals);
+                                  * a triangle with three zero normals is flat */
     const int32_t* mat_ids;      /* [num_tris] index into materials */
     const int32_t* light_ids;    /* [num_tris] index into area lights or -1 */
     int32_t num_materials;

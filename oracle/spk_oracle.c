@@ -144,8 +144,10 @@ static void tri_init(Tri* t, const float* v, const float* nrm) {
     for (int i = 0; i < 3; i++) t->p[i] = v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
     V3 e1 = vsub(t->p[1], t->p[0]);
     V3 e2 = vsub(t->p[2], t->p[0]);
-    if (!nrm) {
-        /* triangle.cc:17-31 */
+    int zero = nrm != NULL;
+    for (int i = 0; nrm && i < 9; i++) zero = zero && nrm[i] == 0.0f;
+    if (!nrm || zero) {
+        /* triangle.cc:17-31 (three zero normals: a flat triangle among smooth ones) */
         t->fn = vnormalized(vcross(e1, e2));
         t->n[0] = t->n[1] = t->n[2] = t->fn;
     } else {

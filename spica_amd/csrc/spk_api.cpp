@@ -98,7 +98,10 @@ TriFrame tri_frame(const float* v, const float* nrm) {
     D3 e1 = dsub(t.p[1], t.p[0]), e2 = dsub(t.p[2], t.p[0]);
     D3 c = dcross(e1, e2);
     t.smooth = false;
-    if (!nrm) {
+    // a triangle whose three normals are all zero is flat (a flat mesh among smooth ones)
+    bool zero = nrm != nullptr;
+    for (int i = 0; nrm && i < 9; i++) zero = zero && nrm[i] == 0.0f;
+    if (!nrm || zero) {
         t.fn = dnormalize(c);
         t.n[0] = t.n[1] = t.n[2] = t.fn;
     } else {

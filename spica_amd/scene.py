@@ -113,6 +113,7 @@ class Scene:
         self._verts = []
         self._mats = []
         self._mat_ids = []
+        self._normals = []
         self._emit = []  # per chunk: radiance or None
         self.materials = []
         self.env = None
@@ -123,10 +124,14 @@ class Scene:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def add_triangles(self, verts, material, radiance=None):
-        """verts: [n, 3, 3]; material: index or dict; radiance: emitter radiance or None."""
+    def add_triangles(self, verts, material, radiance=None, normals=None):
+        """verts: [n, 3, 3]; material: index or dict; radiance: emitter radiance or None;
+        normals: per-corner shading normals [n, 3, 3] (smooth mesh) or None (flat)."""
         v = np.asarray(verts, dtype=np.float32).reshape(-1, 3, 3)
         mid = material if isinstance(material, int) else self.add_material(material)
+        if normals is not None:
+            normals = np.asarray(normals, dtype=np.float32).reshape(v.shape)
+        self._normals.append(normals)
         self._verts.append(v)
         self._mat_ids.append(np.full(len(v), mid, dtype=np.int32))
         self._emit.append(None if radiance is None else np.asarray(radiance, dtype=np.float64).reshape(3))
@@ -165,6 +170,9 @@ class Scene:
             area_tri=np.asarray(area_tri, dtype=np.int32),
             area_radiance=np.ascontiguousarray(np.asarray(area_rad, dtype=np.float64).reshape(-1, 3)),
         )
+        if any(nm is not None for nm in self._normals):
+            fz["normals"] = np.ascontiguousarray(np.concatenate(
+                [np.zeros(v.shape, np.float32) if nm is None else nm for v, nm in zip(self._verts, self._normals)]))
         self._frozen = fz
         return fz
 
@@ -207,7 +215,7 @@ class Scene:
         n = len(fz["verts"])
         d.num_tris = n
         d.verts = fz["verts"].ctypes.data_as(_abi.c_f_p)
-        d.normals = ctypes.cast(None, _abi.c_f_p)
+        d.normals = fz["normals"].ctypes.data_as(_abi.c_f_p) if "normals" in fz else ctypes.cast(None, _abi.c_f_p)
         d.mat_ids = fz["mat_ids"].ctypes.data_as(_abi.c_i32_p)
         d.light_ids = fz["light_ids"].ctypes.data_as(_abi.c_i32_p)
         mats = (_abi.SpkMaterial * max(1, len(self.materials)))()

@@ -1,0 +1,180 @@
+"""spica scene files: the XML parser's rules (sources/spica/sceneparser.cc), the
+Radiance RGBE format (sources/core/image.cc:252-435), PLY/OBJ meshes, and a Cornell box
+written as a scene file that must load into exactly the programmatic scene."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from spica_amd import _abi, configs, sceneio
+from spica_amd.scene import look_at
+
+
+def _write_ply(path, tris):
+    tris = np.asarray(tris, dtype=np.float64).reshape(-1, 3, 3)
+    with open(path, "w") as f:
+        f.write(f"ply\nformat ascii 1.0\nelement vertex {3 * len(tris)}\nproperty float x\nproperty float y\n"
+                f"property float z\nelement face {len(tris)}\nproperty list uchar int vertex_indices\nend_header\n")
+        for t in tris:
+            for v in t:
+                f.write(f"{v[0]:.9g} {v[1]:.9g} {v[2]:.9g}\n")
+        for i in range(len(tris)):
+            f.write(f"3 {3 * i} {3 * i + 1} {3 * i + 2}\n")
+
+
+def _cornell_xml(tmp_path, spp=16):
+    """The C0 Cornell box as a spica scene file: one PLY per triangle group."""
+    w = configs.cornell_box()
+    s = w.scene
+    mats = s.materials
+    lines = ['<?xml version="1.0" encoding="utf-8"?>', '<scene version="0.5.0">',
+             '  <integrator type="path"><integer name="maxDepth" value="8"/></integrator>',
+             '  <sensor type="perspective">', '    <float name="fov" value="40"/>',
+             '    <transform name="toWorld"><lookAt origin="0.5, 0.5, -1.37" target="0.5 0.5 0.0" up="0, 1, 0"/></transform>',
+             f'    <sampler type="independent"><integer name="sampleCount" value="{spp}"/></sampler>',
+             '    <film type="hdrfilm"><integer name="width" value="256"/><integer name="height" value="256"/>'
+             '<rfilter type="box"/></film>', '  </sensor>']
+    for i, m in enumerate(mats):
+        r = m["reflectance"]
+        lines.append(f'  <bsdf type="diffuse" id="m{i}"><rgb name="reflectance" value="{float(r[0])!r}, {float(r[1])!r}, '
+                     f'{float(r[2])!r}"/></bsdf>')
+    for gi, (v, mid, e) in enumerate(zip(s._verts, np.concatenate(s._mat_ids)[np.cumsum([0] + [len(x) for x in s._verts[:-1]])],
+                                         s._emit)):
+        _write_ply(tmp_path / f"g{gi}.ply", v)
+        emit = "" if e is None else (f'<emitter type="area"><rgb name="radiance" value="{float(e[0])!r} {float(e[1])!r} '
+                                     f'{float(e[2])!r}"/></emitter>')
+        lines.append(f'  <shape type="ply"><string name="filename" value="g{gi}.ply"/><ref id="m{mid}"/>{emit}</shape>')
+    lines.append("</scene>")
+    p = tmp_path / "cornell.xml"
+    p.write_text("\n".join(lines))
+    return p, w
+
+
+def test_cornell_scene_file_loads_the_programmatic_scene(tmp_path):
+    p, w = _cornell_xml(tmp_path)
+    xs = sceneio.load_xml(p)
+    a, b = xs.scene.freeze(), w.scene.freeze()
+    assert np.array_equal(a["verts"], b["verts"])
+    assert np.array_equal(a["light_ids"], b["light_ids"]) and np.array_equal(a["area_radiance"], b["area_radiance"])
+    ma = [xs.scene.materials[i]["reflectance"] for i in a["mat_ids"]]
+    mb = [w.scene.materials[i]["reflectance"] for i in b["mat_ids"]]
+    assert ma == mb
+    assert np.allclose(xs.camera.to_world, w.camera.to_world, rtol=0, atol=1e-15)
+    assert (xs.camera.width, xs.camera.height, xs.camera.fov_deg) == (256, 256, 40.0)
+    assert (xs.spp, xs.max_depth, xs.sampler, xs.integrator) == (16, 8, _abi.SPK_SAMPLER_INDEPENDENT, "path")
+
+
+def test_transform_composition_and_rotate_in_radians(tmp_path):
+    p = tmp_path / "t.xml"
+    p.write_text('<scene><transform name="toWorld"><translate x="1" y="2" z="3"/><scale x="2" y="2" z="2"/>'
+                 '<rotate x="0" y="0" z="1" angle="1.5707963267948966"/></transform></scene>')
+    import xml.etree.ElementTree as ET
+    m = sceneio.parse_transform(ET.parse(p).getroot()[0])
+    # applied in order: translate, then scale, then a quarter turn about z (angle taken as radians)
+    pt = m @ np.array([0.0, 0.0, 0.0, 1.0])
+    assert np.allclose(pt[:3], [-4.0, 2.0, 6.0])
+    ex = m @ np.array([1.0, 0.0, 0.0, 0.0])
+    assert np.allclose(ex[:3], [0.0, 2.0, 0.0])
+    r = sceneio.rotate(0.3, [0, 0, 2])
+    assert np.allclose(r[:2, :2], [[math.cos(0.3), -math.sin(0.3)], [math.sin(0.3), math.cos(0.3)]])
+    la = sceneio.parse_transform(ET.fromstring('<t><lookAt origin="1 2 3" target="1 2 4" up="0 1 0"/></t>'))
+    assert np.allclose(la, look_at((1, 2, 3), (1, 2, 4), (0, 1, 0)))
+
+
+def test_vector_parsing():
+    assert np.allclose(sceneio.parse_vec3("1 2 3"), [1, 2, 3])
+    assert np.allclose(sceneio.parse_vec3("1, 2, 3"), [1, 2, 3])
+    assert np.allclose(sceneio.parse_vec3("0.5"), [0.5, 0.5, 0.5])
+    with pytest.raises(ValueError):
+        sceneio.parse_vec3("a b c")
+
+
+def test_rgbe_known_answer_and_round_trip(tmp_path):
+    img = np.zeros((3, 5, 3), dtype=np.float32)
+    img[0, 0] = (1.0, 0.5, 0.25)       # mantissa 0.5, exponent 1: bytes (128, 64, 32, 129)
+    img[1, 2] = (3.0e4, 10.0, 1e-3)
+    img[2, 4] = (0.0, 0.0, 0.0)
+    path = tmp_path / "a.hdr"
+    sceneio.write_rgbe(path, img)
+    raw = path.read_bytes()
+    body = raw[raw.index(b"+X 5\n") + 5:]
+    assert body[:4] == bytes([2, 2, 0, 5])
+    assert [body[5], body[11], body[17], body[23]] == [128, 64, 32, 129]
+    back = sceneio.read_rgbe(path)
+    assert back.shape == img.shape
+    assert np.array_equal(back[0, 0], img[0, 0])
+    big = img.max(axis=2, keepdims=True)
+    assert np.all(np.abs(back - img) <= big / 128.0 + 1e-30)
+
+
+def test_rgbe_reads_run_length_runs(tmp_path):
+    # a scanline of 4 equal pixels written with runs (count > 128 = repeat)
+    hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 4\n"
+    line = bytes([2, 2, 0, 4]) + b"".join(bytes([128 + 4, v]) for v in (128, 128, 128, 130))
+    path = tmp_path / "r.hdr"
+    path.write_bytes(hdr + line)
+    assert np.allclose(sceneio.read_rgbe(path), 2.0)
+
+
+def test_obj_fans_polygons_and_keeps_normals(tmp_path):
+    p = tmp_path / "q.obj"
+    p.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1 4//1\n")
+    tris, nrm = sceneio.load_obj(p, to_world=np.diag([2.0, 2.0, 2.0, 1.0]))
+    assert tris.shape == (2, 3, 3) and np.allclose(tris[1, 2], [0, 2, 0])
+    assert nrm.shape == (2, 3, 3) and np.allclose(nrm[..., 2] / np.linalg.norm(nrm, axis=-1), 1.0)
+
+
+def test_unsupported_plugins_raise(tmp_path):
+    p = tmp_path / "bad.xml"
+    p.write_text('<scene><integrator type="bdpt"/></scene>')
+    with pytest.raises(ValueError, match="bdpt"):
+        sceneio.load_xml(p)
+    p.write_text('<scene><bsdf type="roughconductor" id="r"><rgb name="eta" value="1"/><rgb name="k" value="1"/>'
+                 '</bsdf></scene>')
+    with pytest.raises(ValueError, match="beckmann"):
+        sceneio.load_xml(p)
+
+
+def test_envmap_emitter_loads_rgbe(tmp_path):
+    sky = configs.sky_envmap(32, 16, sun=100.0)
+    sceneio.write_rgbe(tmp_path / "sky.hdr", sky)
+    p, _ = _cornell_xml(tmp_path)
+    text = p.read_text().replace("</scene>", '<emitter type="envmap"><string name="filename" value="sky.hdr"/>'
+                                               '<float name="scale" value="2"/><float name="worldRadius" value="9"/></emitter></scene>')
+    p.write_text(text)
+    xs = sceneio.load_xml(p)
+    assert xs.scene.env is not None and xs.scene.env["rgb"].shape == (16, 32, 3)
+    assert xs.scene.env["scale"] == 2.0 and xs.scene.env["radius"] == 9.0
+    assert np.allclose(xs.scene.env["rgb"], sky, rtol=1 / 64)
+
+
+@pytest.mark.gpu
+def test_scene_file_renders_like_the_programmatic_scene(gpu, tmp_path):
+    p, w = _cornell_xml(tmp_path, spp=8)
+    xs = sceneio.load_xml(p)
+    films = []
+    for wl in (xs.workload(), configs.Workload("c", w.scene, w.camera, 8, 8, _abi.SPK_SAMPLER_INDEPENDENT, 0)):
+        gs = gpu.GpuScene(wl.scene)
+        gr = gpu.GpuRender(gs, wl.camera, wl.render_desc())
+        gr.run(8)
+        films.append(gr.film())
+        gr.close()
+        gs.close()
+    for a, b in zip(*films):
+        assert np.array_equal(a, b)
+
+
+def test_zero_normals_mark_flat_triangles(oracle):
+    """A normals array whose entries are zero for some triangles leaves those triangles
+    flat: the checker renders the same image as without any normals."""
+    w = configs.cornell_box(width=24, height=24)
+    s2 = configs.cornell_box(width=24, height=24).scene
+    s2._normals = [np.zeros(v.shape, np.float32) for v in s2._verts]
+    s2._frozen = None
+    assert "normals" in s2.freeze()
+    rd = w.render_desc(spp=4)
+    a = oracle.scene(w.scene).render(w.camera, rd, threads=4)
+    b = oracle.scene(s2).render(w.camera, rd, threads=4)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
