@@ -132,6 +132,23 @@ struct Counters {
     uint32_t nodes, tris;
 };
 
+// Both child boxes of a node against the ray, in FMA form: plane t = p * (1/d) - o * (1/d)
+// (one fused multiply-add per plane), entry clamped at 0 and exit at the current best t.
+// Node layout: a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y), b = same for R, c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z).
+__device__ __forceinline__ void slab2(float4 a, float4 b, float4 c, float3 inv, float3 oinv, float tbest, float& tn0,
+                                      float& tf0, float& tn1, float& tf1) {
+    const float l0x = fmaf(a.x, inv.x, -oinv.x), h0x = fmaf(a.y, inv.x, -oinv.x);
+    const float l0y = fmaf(a.z, inv.y, -oinv.y), h0y = fmaf(a.w, inv.y, -oinv.y);
+    const float l0z = fmaf(c.x, inv.z, -oinv.z), h0z = fmaf(c.y, inv.z, -oinv.z);
+    const float l1x = fmaf(b.x, inv.x, -oinv.x), h1x = fmaf(b.y, inv.x, -oinv.x);
+    const float l1y = fmaf(b.z, inv.y, -oinv.y), h1y = fmaf(b.w, inv.y, -oinv.y);
+    const float l1z = fmaf(c.z, inv.z, -oinv.z), h1z = fmaf(c.w, inv.z, -oinv.z);
+    tn0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), 0.0f));
+    tf0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tbest));
+    tn1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), 0.0f));
+    tf1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tbest));
+}
+
 // Closest hit (ANY = false) or first hit (ANY = true).  Returns the leaf-order
 // triangle index or -1.  `col` is this lane's LDS stack column (slot s at
 // col[s * BLOCK]); `spill` holds entries below the S most recent ones.
@@ -141,6 +158,7 @@ __device__ __forceinline__ int32_t traverse(const SceneDev& sc, float3 o, float3
     const float4* __restrict__ nodes = sc.nodes;
     const float4* __restrict__ tris = sc.tris;
     float3 inv = safe_inv(d);
+    const float3 oinv = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
     int32_t node = (active && sc.has_bvh) ? 0 : kSent;
     int32_t leaf = 0;  // pending leaf link (negative) or 0
     int sp = 0, spilled = 0;
@@ -172,16 +190,8 @@ __device__ __forceinline__ int32_t traverse(const SceneDev& sc, float3 o, float3
             float4 a = nd[0], b = nd[1], c = nd[2];
             float4 lk = nd[3];
             if (COUNT) cnt.nodes++;
-            float l0x = (a.x - o.x) * inv.x, h0x = (a.y - o.x) * inv.x;
-            float l0y = (a.z - o.y) * inv.y, h0y = (a.w - o.y) * inv.y;
-            float l0z = (c.x - o.z) * inv.z, h0z = (c.y - o.z) * inv.z;
-            float l1x = (b.x - o.x) * inv.x, h1x = (b.y - o.x) * inv.x;
-            float l1y = (b.z - o.y) * inv.y, h1y = (b.w - o.y) * inv.y;
-            float l1z = (c.z - o.z) * inv.z, h1z = (c.w - o.z) * inv.z;
-            float tn0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), 0.0f));
-            float tf0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tbest));
-            float tn1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), 0.0f));
-            float tf1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tbest));
+            float tn0, tf0, tn1, tf1;
+            slab2(a, b, c, inv, oinv, tbest, tn0, tf0, tn1, tf1);
             bool h0 = tn0 <= tf0 * kRobust;
             bool h1 = tn1 <= tf1 * kRobust;
             int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y);
@@ -258,6 +268,152 @@ __device__ __forceinline__ WaveSplit wave_split() {
     s.first = wave * 64u;
     s.stride = g * (uint32_t)BLOCK;
     return s;
+}
+
+// Streaming traversal for the persistent queue kernels: every lane traces its own ray
+// and, when at least kRefill lanes of the wave are idle, the wave hands the next rays of
+// its slice (the same 64-ray chunks wave_split assigns, in order) to the idle lanes, so
+// a wave is not held by its slowest ray (dynamic ray fetch, Aila and Laine 2009).
+// load(i, o, d, tmax) reads ray i and returns its scene (batch: an entry of the scene
+// table; single: null, the kernel's scene argument is used); done(i, hit, t, u, v, scene)
+// consumes its result.
+#ifndef SPK_REFILL
+#define SPK_REFILL 8
+#endif
+constexpr int kRefill = SPK_REFILL;
+
+template <bool BATCH>
+__device__ __forceinline__ const SceneDev& pick_scene(const SceneDev* p, const SceneDev& s) {
+    if constexpr (BATCH) return *p;
+    else return s;
+}
+
+template <int BLOCK, int S, bool ANY, bool COUNT, bool BATCH, class Load, class Done>
+__device__ __forceinline__ void traverse_stream(const SceneDev& sc0, const SceneDev* scenes, uint32_t count, Load load,
+                                                Done done, uint32_t* col, uint32_t* spill, Counters& cnt) {
+    const WaveSplit ws = wave_split<BLOCK>();
+    bool active = false;
+    uint32_t idx = 0;
+    const SceneDev* sc = scenes;  // batch only: this lane's scene
+    float3 o = mk(0, 0, 0), d = mk(0, 0, 1), inv = mk(1, 1, 1), oinv = mk(0, 0, 0);
+    int32_t node = kSent, leaf = 0;
+    int sp = 0, spilled = 0;
+    int32_t hit = -1;
+    float tbest = 0.0f, hu = 0.0f, hv = 0.0f;
+    uint32_t cursor = 0;  // wave-uniform: positions of the wave's slice handed out so far
+    auto index_of = [&](uint32_t pos) { return ws.first + (pos >> 6) * ws.stride + (pos & 63u); };
+    bool exhausted = index_of(0) >= count;
+
+    auto push = [&](int32_t x) {
+        if (sp - spilled == S) {
+            spill[spilled] = col[(spilled & (S - 1)) * BLOCK];
+            spilled++;
+        }
+        col[(sp & (S - 1)) * BLOCK] = (uint32_t)x;
+        sp++;
+    };
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kSent;
+        sp--;
+        if (sp < spilled) {
+            spilled = sp;
+            return (int32_t)spill[sp];
+        }
+        return (int32_t)col[(sp & (S - 1)) * BLOCK];
+    };
+
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            if (n_idle >= (uint32_t)kRefill || n_idle == 64u) {
+                if (!active) {
+                    const uint32_t i = index_of(cursor + (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull)));
+                    if (i < count) {
+                        float tmax;
+                        const SceneDev* ls = load(i, o, d, tmax);
+                        if constexpr (BATCH) sc = ls;
+                        idx = i;
+                        active = true;
+                        inv = safe_inv(d);
+                        oinv = mk(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+                        node = pick_scene<BATCH>(sc, sc0).has_bvh ? 0 : kSent;
+                        leaf = 0;
+                        sp = 0;
+                        spilled = 0;
+                        hit = -1;
+                        tbest = tmax;
+                        hu = hv = 0.0f;
+                    }
+                }
+                cursor += n_idle;
+                exhausted = index_of(cursor) >= count;
+            }
+        }
+        if (!__any(active)) break;
+        const float4* __restrict__ nodes = pick_scene<BATCH>(sc, sc0).nodes;
+        const float4* __restrict__ tris = pick_scene<BATCH>(sc, sc0).tris;
+        // Interior nodes until every busy lane holds a leaf (or is done).
+        while (node >= 0 && node != kSent) {
+            const float4* nd = nodes + 4 * node;
+            float4 a = nd[0], b = nd[1], c = nd[2];
+            float4 lk = nd[3];
+            if (COUNT) cnt.nodes++;
+            float tn0, tf0, tn1, tf1;
+            slab2(a, b, c, inv, oinv, tbest, tn0, tf0, tn1, tf1);
+            bool h0 = tn0 <= tf0 * kRobust;
+            bool h1 = tn1 <= tf1 * kRobust;
+            int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y);
+            if (h0 && h1) {
+                bool first0 = tn0 <= tn1;
+                node = first0 ? c0 : c1;
+                push(first0 ? c1 : c0);
+            } else if (h0) {
+                node = c0;
+            } else if (h1) {
+                node = c1;
+            } else {
+                node = pop();
+            }
+            if (node < 0 && leaf == 0) {
+                leaf = node;
+                node = pop();
+            }
+            if (__all(leaf != 0)) break;
+        }
+        // Leaves.
+        while (leaf < 0) {
+            uint32_t x = ~(uint32_t)leaf;
+            uint32_t first = x >> kLeafCountBits, n = (x & (kMaxLeafTris - 1)) + 1;
+            for (uint32_t k = first; k < first + n; k++) {
+                float t, u, v;
+                if (COUNT) cnt.tris++;
+                if (tri_hit(tris, k, o, d, tbest, t, u, v)) {
+                    hit = (int32_t)k;
+                    if (ANY) break;
+                    tbest = t;
+                    hu = u;
+                    hv = v;
+                }
+            }
+            if (ANY && hit >= 0) {
+                node = kSent;
+                leaf = 0;
+                sp = 0;
+                spilled = 0;
+                break;
+            }
+            leaf = 0;
+            if (node < 0) {
+                leaf = node;
+                node = pop();
+            }
+        }
+        if (active && node == kSent && leaf == 0) {
+            done(idx, hit, tbest, hu, hv, pick_scene<BATCH>(sc, sc0));
+            active = false;
+        }
+    }
 }
 
 // ---- standalone trace (spk_trace) --------------------------------------------------------
@@ -475,27 +631,23 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, in
     const float4* __restrict__ qo = rb.qa_o[depth & 1];
     const float4* __restrict__ qd = rb.qa_d[depth & 1];
     const float4* __restrict__ qb = rb.qa_b[depth & 1];
-    const WaveSplit ws = wave_split<BLOCK>();
-    for (uint32_t base = ws.first; base < count; base += ws.stride) {
-        uint32_t i = base + __lane_id();
-        bool active = i < count;
-        float3 o = mk(0, 0, 0), d = mk(0, 0, 1);
-        uint32_t img = 0;
-        if (active) {
-            o = xyz(qo[i]);
-            d = xyz(qd[i]);
-            if (BATCH) img = batch_image(__float_as_uint(qb[i].w));
-        }
-        const SceneDev& s = BATCH ? rb.scenes[img] : sc;
-        float t, u, v;
-        Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, false, COUNT>(s, o, d, kINFTY, t, u, v, col, spill, active, cnt);
-        if (active) rb.hit[i] = make_float4(t, u, v, __int_as_float(k));
-        if (COUNT && active) {
-            atomicAdd(&ps->count_nodes, (unsigned long long)cnt.nodes);
-            atomicAdd(&ps->count_tris, (unsigned long long)cnt.tris);
-            atomicAdd(&ps->count_rays, 1ull);
-        }
+    Counters cnt = {0, 0};
+    uint32_t rays = 0;
+    auto load = [&](uint32_t i, float3& o, float3& d, float& tmax) -> const SceneDev* {
+        o = xyz(qo[i]);
+        d = xyz(qd[i]);
+        tmax = kINFTY;
+        return BATCH ? rb.scenes + batch_image(__float_as_uint(qb[i].w)) : nullptr;
+    };
+    auto done = [&](uint32_t i, int32_t k, float t, float u, float v, const SceneDev&) {
+        rb.hit[i] = make_float4(t, u, v, __int_as_float(k));
+        if (COUNT) rays++;
+    };
+    traverse_stream<BLOCK, S, false, COUNT, BATCH>(sc, rb.scenes, count, load, done, col, spill, cnt);
+    if (COUNT) {
+        atomicAdd(&ps->count_nodes, (unsigned long long)cnt.nodes);
+        atomicAdd(&ps->count_tris, (unsigned long long)cnt.tris);
+        atomicAdd(&ps->count_rays, (unsigned long long)rays);
     }
 }
 
@@ -1149,27 +1301,23 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
-    const WaveSplit ws = wave_split<BLOCK>();
-    for (uint32_t base = ws.first; base < count; base += ws.stride) {
-        uint32_t i = base + __lane_id();
-        bool active = i < count;
-        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0), C = make_float4(0, 0, 0, 0);
-        if (active) {
-            O = rb.qs_o[i];
-            D = rb.qs_d[i];
-            if (BATCH) C = rb.qs_c[i];
-        }
-        const SceneDev& s = BATCH ? rb.scenes[__float_as_uint(C.w)] : sc;
-        float t, u, v;
-        Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, true, false>(s, xyz(O), xyz(D), D.w, t, u, v, col, spill, active, cnt);
-        if (active && k < 0) {
-            uint32_t path = __float_as_uint(O.w);
-            float4 c = BATCH ? C : rb.qs_c[i];
+    Counters cnt = {0, 0};
+    auto load = [&](uint32_t i, float3& o, float3& d, float& tmax) -> const SceneDev* {
+        const float4 O = rb.qs_o[i], D = rb.qs_d[i];
+        o = xyz(O);
+        d = xyz(D);
+        tmax = D.w;
+        return BATCH ? rb.scenes + __float_as_uint(rb.qs_c[i].w) : nullptr;
+    };
+    auto done = [&](uint32_t i, int32_t k, float, float, float, const SceneDev&) {
+        if (k < 0) {
+            const uint32_t path = __float_as_uint(rb.qs_o[i].w);
+            const float4 c = rb.qs_c[i];
             float4 Lv = rb.path_L[path];
             rb.path_L[path] = make_float4(Lv.x + c.x, Lv.y + c.y, Lv.z + c.z, Lv.w);
         }
-    }
+    };
+    traverse_stream<BLOCK, S, true, false, BATCH>(sc, rb.scenes, count, load, done, col, spill, cnt);
 }
 
 // ---- MIS: NEE BSDF-sample ray; counts if it reaches the chosen light (mis.cc:114-133) ----------
@@ -1181,37 +1329,31 @@ __global__ __launch_bounds__(BLOCK) void k_mis(SceneDev sc, RenderBufs rb, int d
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
-    const WaveSplit ws = wave_split<BLOCK>();
-    for (uint32_t base = ws.first; base < count; base += ws.stride) {
-        uint32_t i = base + __lane_id();
-        bool active = i < count;
-        float4 O = make_float4(0, 0, 0, 0), D = make_float4(0, 0, 1, 0), C = make_float4(0, 0, 0, 0);
-        if (active) {
-            O = rb.qm_o[i];
-            D = rb.qm_d[i];
-            if (BATCH) C = rb.qm_c[i];
+    Counters cnt = {0, 0};
+    auto load = [&](uint32_t i, float3& o, float3& d, float& tmax) -> const SceneDev* {
+        o = xyz(rb.qm_o[i]);
+        d = xyz(rb.qm_d[i]);
+        tmax = kINFTY;
+        return BATCH ? rb.scenes + __float_as_uint(rb.qm_c[i].w) : nullptr;
+    };
+    auto done = [&](uint32_t i, int32_t k, float, float, float, const SceneDev& s) {
+        const float4 D = rb.qm_d[i];
+        const int li = __float_as_int(D.w);
+        float3 Le = mk(0, 0, 0);
+        if (k >= 0) {
+            if (li < s.num_areas && s.tri_ids[k].y == li && dot(xyz(s.tri_ng[k]), neg(xyz(D))) > 0.0f)
+                Le = xyz(s.lights[li].L);
+        } else if (li >= s.num_areas) {
+            Le = env_Le(s.env, xyz(D));
         }
-        const SceneDev& s = BATCH ? rb.scenes[__float_as_uint(C.w)] : sc;
-        float t, u, v;
-        Counters cnt = {0, 0};
-        int32_t k = traverse<BLOCK, S, false, false>(s, xyz(O), xyz(D), kINFTY, t, u, v, col, spill, active, cnt);
-        if (active) {
-            int li = __float_as_int(D.w);
-            float3 Le = mk(0, 0, 0);
-            if (k >= 0) {
-                if (li < s.num_areas && s.tri_ids[k].y == li && dot(xyz(s.tri_ng[k]), neg(xyz(D))) > 0.0f)
-                    Le = xyz(s.lights[li].L);
-            } else if (li >= s.num_areas) {
-                Le = env_Le(s.env, xyz(D));
-            }
-            if (!black(Le)) {
-                uint32_t path = __float_as_uint(O.w);
-                float4 c = BATCH ? C : rb.qm_c[i];
-                float4 Lv = rb.path_L[path];
-                rb.path_L[path] = make_float4(Lv.x + c.x * Le.x, Lv.y + c.y * Le.y, Lv.z + c.z * Le.z, Lv.w);
-            }
+        if (!black(Le)) {
+            const uint32_t path = __float_as_uint(rb.qm_o[i].w);
+            const float4 c = rb.qm_c[i];
+            float4 Lv = rb.path_L[path];
+            rb.path_L[path] = make_float4(Lv.x + c.x * Le.x, Lv.y + c.y * Le.y, Lv.z + c.z * Le.z, Lv.w);
         }
-    }
+    };
+    traverse_stream<BLOCK, S, false, false, BATCH>(sc, rb.scenes, count, load, done, col, spill, cnt);
 }
 
 // ---- guide: learned continuation proposal ---------------------------------------------------------
