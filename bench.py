@@ -323,8 +323,8 @@ def main():
                     gr.set_guide(g)
                 t0 = time.perf_counter()
                 gr.run(args.heavy_spp, sync=True)
-                dt = time.perf_counter() - t0
-                res[name] = dict(spp=args.heavy_spp, seconds=round(dt, 3),
+                vdt = time.perf_counter() - t0
+                res[name] = dict(spp=args.heavy_spp, seconds=round(vdt, 3),
                                  relmse=float(f"{gr.relmse_pixels(pix, refm):.5g}"))
                 gr.close()
                 if g is not None:
