@@ -75,7 +75,15 @@ def cpu_baseline(w, threads, budget_s):
         rays += int(st[0])
         paths += n
         n = int(min(200000, max(2000, n * max(1.0, 0.5 * (budget_s - elapsed) / max(dt, 1e-3)))))
-    return dict(value=rays / elapsed / 1e6, unit="Mrays/s", cores=threads, kind="port",
+    cal = None
+    cal_path = os.path.join(ROOT, "oracle", "calibration.json")
+    if os.path.exists(cal_path):
+        with open(cal_path) as f:
+            c = json.load(f)
+        r = c["spica_over_restatement_1thread"]
+        cal = dict(spica_over_port=r, spica_equivalent_mrays_per_s=round(r * rays / elapsed / 1e6, 3),
+                   measured=c["workload"] + "; " + c["measured"])
+    return dict(value=rays / elapsed / 1e6, unit="Mrays/s", cores=threads, kind="port", calibration=cal,
                 sample=f"{paths} random pixels of the 1920x1080 frame at 1 spp ({rays} rays, {elapsed:.1f} s); "
                        f"spica-style BVH build {build_s:.1f} s excluded",
                 mpaths_per_s=paths / elapsed / 1e6)
