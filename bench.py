@@ -312,11 +312,28 @@ def main():
         brays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
         bpaths = st1["paths"] - st0["paths"]
         b.close()
+        # the same batch over a linear BVH built on the device
+        gl = render.GpuScene(views[0].scene, builder="lbvh")
+        stl = gl.stats()
+        bl = render.Batch([(gl,) + it[1:] for it in items], max_depth=views[0].max_depth, sampler=views[0].sampler,
+                          queue_size=args.queue, block_size=args.block, graph=1)
+        bl.run()
+        sl0 = bl.stats()
+        t0 = time.perf_counter()
+        bl.run(sync=True)
+        ldt = time.perf_counter() - t0
+        sl1 = bl.stats()
+        lrays = sum(sl1[k] - sl0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+        bl.close()
+        gl.close()
         heavy = dict(workload="C4: seed 11, 6000 spheres + 2 noise-displaced 1M-triangle grids, 1024x512 HDR sky "
                               "(sun 1e4, gradient 0.5-2.0), 4 quad lights; 8 views 1920x1080, maxDepth 32, Sobol",
                      triangles=int(st4["num_tris"]), bvh_build_ms=round(st4["build_ms"], 1), setup_s=round(setup_s, 1),
                      batch=dict(views=len(items), spp=vspp, seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
-                                mpaths_per_s=round(bpaths / bdt / 1e6, 3), rays_per_path=round(brays / bpaths, 3)))
+                                mpaths_per_s=round(bpaths / bdt / 1e6, 3), rays_per_path=round(brays / bpaths, 3)),
+                     lbvh=dict(bvh_build_ms=round(stl["build_ms"], 1), nodes=int(stl["num_nodes"]),
+                               max_depth=int(stl["max_depth"]), batch_seconds=round(ldt, 3),
+                               mrays_per_s=round(lrays / ldt / 1e6, 3)))
         c4_ref = load_reference("c4")
         if c4_ref is not None:
             pix, refm, rspp = c4_ref

@@ -51,6 +51,12 @@ int spk_init(int device);
  * together with the per-primitive material/light binding of
  * SceneParser::createPrimitive (sources/spica/sceneparser.cc:168-188). */
 int spk_scene_create(const spk_scene_desc* desc, spk_scene** out);
+/* The same, with a choice of builder and leaf size (opts NULL = defaults: SAH, 4).
+ * spica creates its accelerator from the "accelerator" render parameter, fixed to
+ * "bvh" (sources/spica/sceneparser.cc:64, 91-95 -> accelerators/bvh.cc:112-148); the
+ * LBVH builder runs on the device instead of the host SAH build. */
+int spk_scene_build_opts_default(spk_scene_build_opts* opts);
+int spk_scene_create_ex(const spk_scene_desc* desc, const spk_scene_build_opts* opts, spk_scene** out);
 int spk_scene_destroy(spk_scene* scene);
 
 typedef struct spk_scene_stats {
@@ -59,7 +65,7 @@ typedef struct spk_scene_stats {
     int64_t num_leaves;
     int32_t max_depth;
     int32_t num_lights;
-    double build_ms;        /* host BVH build */
+    double build_ms;        /* BVH build (host SAH, or device LBVH with its transfers) */
     double upload_ms;
     int64_t device_bytes;
 } spk_scene_stats;

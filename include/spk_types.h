@@ -68,6 +68,20 @@ typedef struct spk_scene_desc {
     double env_radius;           /* "worldRadius" (spica default 2.0) */
 } spk_scene_desc;
 
+/* Acceleration-structure builder.  spica builds one binned-SAH BVH on the CPU
+ * (accelerators/bvh.cc:112-148, the "accelerator" parameter "bvh"); SPK_BVH_LBVH builds a linear BVH
+ * on the device instead (Morton order, radix tree), cheaper to build and somewhat
+ * slower to trace.  Both give the same closest hits. */
+enum spk_bvh_builder {
+    SPK_BVH_SAH = 0,
+    SPK_BVH_LBVH = 1
+};
+
+typedef struct spk_scene_build_opts {
+    int32_t builder;             /* enum spk_bvh_builder */
+    int32_t max_leaf;            /* triangles per leaf, 1..8 (spica: 4) */
+} spk_scene_build_opts;
+
 /* Perspective camera (cameras/perspective.cc:23-74): "toWorld", "fov" (degrees,
  * vertical), "apertureRadius", "focusDistance", film "width"/"height". */
 typedef struct spk_camera_desc {

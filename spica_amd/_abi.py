@@ -18,6 +18,9 @@ SPK_SAMPLER_INDEPENDENT = 0
 SPK_SAMPLER_SOBOL = 1
 SPK_SAMPLER_MT19937 = 2
 
+SPK_BVH_SAH = 0
+SPK_BVH_LBVH = 1
+
 SPK_OK = 0
 SPK_ERR_INVALID = 1
 SPK_ERR_HIP = 2
@@ -85,6 +88,10 @@ class SpkRenderDesc(ctypes.Structure):
         ("rr_depth", ctypes.c_int32),
         ("first_sample", ctypes.c_int32),
     ]
+
+
+class SpkSceneBuildOpts(ctypes.Structure):
+    _fields_ = [("builder", ctypes.c_int32), ("max_leaf", ctypes.c_int32)]
 
 
 class SpkSceneStats(ctypes.Structure):
@@ -192,6 +199,9 @@ SIGNATURES = {
     "spk_version": (ctypes.c_char_p, []),
     "spk_init": (ctypes.c_int, [ctypes.c_int]),
     "spk_scene_create": (ctypes.c_int, [ctypes.POINTER(SpkSceneDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "spk_scene_build_opts_default": (ctypes.c_int, [ctypes.POINTER(SpkSceneBuildOpts)]),
+    "spk_scene_create_ex": (ctypes.c_int, [ctypes.POINTER(SpkSceneDesc), ctypes.POINTER(SpkSceneBuildOpts),
+                                           ctypes.POINTER(ctypes.c_void_p)]),
     "spk_scene_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "spk_scene_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkSceneStats)]),
     "spk_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
@@ -244,6 +254,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise SpkError(f"{LIB_PATH} is missing: build the HIP extension first (python -c 'import __graft_entry__ as g; g.build()')")
+    # torch ships its own HIP runtime.  Imported first, its libamdhip64 satisfies the
+    # library's dependency and the process holds one runtime; loading the library first
+    # brings in the system runtime as a second one, and whichever of the two initialises
+    # later finds no device.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

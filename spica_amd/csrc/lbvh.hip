@@ -1,0 +1,273 @@
+// lbvh.hip -- linear BVH built on the GPU (Karras, "Maximizing parallelism in the
+// construction of BVHs, octrees, and k-d trees", HPG 2012).
+//
+//   1. 30-bit Morton code of each triangle centroid in the scene box, sorted with its
+//      index (hipCUB radix sort);
+//   2. one thread per internal node finds the node's key range and split from the
+//      common-prefix lengths of neighbouring codes (ties broken by index), giving a
+//      binary radix tree whose node i covers a contiguous range of sorted triangles;
+//   3. boxes bottom-up: each leaf walks to the root, the second thread to reach a node
+//      (an atomic counter per node) merges its children's boxes and computes the node's
+//      SAH cost; a subtree of at most max_leaf triangles whose cost as one leaf is no
+//      higher than as a subtree is marked to collapse (the rule of Karras and Aila, "Fast
+//      parallel construction of high-quality BVHs", HPG 2013);
+//   4. emit the 64-byte nodes the traversal reads, collapsed subtrees as leaves over their
+//      contiguous ranges, then renumber the reachable nodes depth first on the host.
+// Node 0 is the root, as the traversal expects.  The tree answers the same closest hits
+// as the SAH tree (only exact-tie winners may differ); it is cheaper to build and somewhat
+// slower to traverse.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "spk_internal.h"
+
+namespace spk {
+
+namespace {
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const float* __restrict__ v, uint32_t n, float3 lo, float3 scale, uint32_t* codes,
+                         uint32_t* idx) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* t = v + 9 * (size_t)i;
+        const float cx = (t[0] + t[3] + t[6]) * (1.0f / 3.0f);
+        const float cy = (t[1] + t[4] + t[7]) * (1.0f / 3.0f);
+        const float cz = (t[2] + t[5] + t[8]) * (1.0f / 3.0f);
+        const uint32_t x = (uint32_t)fminf(fmaxf((cx - lo.x) * scale.x, 0.0f), 1023.0f);
+        const uint32_t y = (uint32_t)fminf(fmaxf((cy - lo.y) * scale.y, 0.0f), 1023.0f);
+        const uint32_t z = (uint32_t)fminf(fmaxf((cz - lo.z) * scale.z, 0.0f), 1023.0f);
+        codes[i] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+        idx[i] = i;
+    }
+}
+
+// Common-prefix length of the keys at i and j (the index breaks ties), -1 out of range.
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ c, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = c[i], b = c[j];
+    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clz(a ^ b);
+}
+
+// Internal nodes 0 .. n-2; child ids < n-1 are internal nodes, id (n-1) + k is leaf k.
+__global__ void k_hierarchy(const uint32_t* __restrict__ c, int n, int2* child, int2* range, int* parent) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
+        const int d = (delta(c, n, i, i + 1) - delta(c, n, i, i - 1)) >= 0 ? 1 : -1;
+        const int dmin = delta(c, n, i, i - d);
+        int lmax = 2;
+        while (delta(c, n, i, i + lmax * d) > dmin) lmax *= 2;
+        int l = 0;
+        for (int t = lmax / 2; t >= 1; t /= 2)
+            if (delta(c, n, i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(c, n, i, j);
+        int s = 0, t = l;
+        do {
+            t = (t + 1) / 2;
+            if (delta(c, n, i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        const int gamma = i + s * d + (d < 0 ? -1 : 0);
+        const int lo = min(i, j), hi = max(i, j);
+        const int left = lo == gamma ? (n - 1) + gamma : gamma;
+        const int right = hi == gamma + 1 ? (n - 1) + gamma + 1 : gamma + 1;
+        child[i] = make_int2(left, right);
+        range[i] = make_int2(lo, hi);
+        parent[left] = i;
+        parent[right] = i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) parent[0] = -1;
+}
+
+struct BoxF {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ float half_area(const BoxF& b) {
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Boxes and SAH costs bottom-up.  cost = node_cost * A + cost(left) + cost(right) for an
+// interior node, tri_cost * count * A for a leaf; a subtree of at most max_leaf triangles
+// whose leaf cost is no higher becomes one leaf (collapse[p] = 1).
+__global__ void k_refit(const float* __restrict__ v, const uint32_t* __restrict__ order, int n,
+                        const int2* __restrict__ child, const int2* __restrict__ range, const int* __restrict__ parent,
+                        BoxF* box, float* cost, uint32_t* collapse, uint32_t* flag, int max_leaf, float cn, float ct) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const float* t = v + 9 * (size_t)order[k];
+        BoxF b;
+        for (int a = 0; a < 3; a++) {
+            b.lo[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+            b.hi[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+        }
+        int node = (n - 1) + k;
+        box[node] = b;
+        cost[node] = ct * half_area(b);
+        __threadfence();
+        int p = parent[node];
+        while (p >= 0) {
+            // the first thread to arrive leaves; the second has both children's results
+            if (atomicAdd(&flag[p], 1u) == 0u) break;
+            __threadfence();
+            const int2 ch = child[p];
+            const BoxF l = box[ch.x], r = box[ch.y];
+            BoxF m;
+            for (int a = 0; a < 3; a++) {
+                m.lo[a] = fminf(l.lo[a], r.lo[a]);
+                m.hi[a] = fmaxf(l.hi[a], r.hi[a]);
+            }
+            const float area = half_area(m);
+            const float c_int = cn * area + cost[ch.x] + cost[ch.y];
+            const int2 rg = range[p];
+            const int cnt = rg.y - rg.x + 1;
+            const float c_leaf = cnt <= max_leaf ? ct * (float)cnt * area : 3.0e38f;
+            box[p] = m;
+            cost[p] = fminf(c_int, c_leaf);
+            collapse[p] = c_leaf <= c_int ? 1u : 0u;
+            __threadfence();
+            p = parent[p];
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t leaf_link_dev(uint32_t first, uint32_t count) {
+    return ~(int32_t)((first << kLeafCountBits) | (count - 1));
+}
+
+__global__ void k_emit(const int2* __restrict__ child, const int2* __restrict__ range, const BoxF* __restrict__ box, int n,
+                       const uint32_t* __restrict__ collapse, BvhNodeHost* out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
+        const int2 ch = child[i];
+        const BoxF l = box[ch.x], r = box[ch.y];
+        BvhNodeHost h;
+        h.a[0] = l.lo[0]; h.a[1] = l.hi[0]; h.a[2] = l.lo[1]; h.a[3] = l.hi[1];
+        h.b[0] = r.lo[0]; h.b[1] = r.hi[0]; h.b[2] = r.lo[1]; h.b[3] = r.hi[1];
+        h.c[0] = l.lo[2]; h.c[1] = l.hi[2]; h.c[2] = r.lo[2]; h.c[3] = r.hi[2];
+        const int ids[2] = {ch.x, ch.y};
+        for (int side = 0; side < 2; side++) {
+            const int c = ids[side];
+            if (c >= n - 1) {
+                h.d[side] = leaf_link_dev((uint32_t)(c - (n - 1)), 1u);
+            } else {
+                const int2 rg = range[c];
+                h.d[side] = collapse[c] ? leaf_link_dev((uint32_t)rg.x, (uint32_t)(rg.y - rg.x + 1)) : c;
+            }
+        }
+        h.d[2] = h.d[3] = 0;
+        out[i] = h;
+    }
+}
+
+struct Bufs {
+    DevBuf dv, codes, idx, codes_s, idx_s, tmp, child, range, parent, box, cost, collapse, flag, nodes;
+    ~Bufs() {
+        for (DevBuf* b : {&dv, &codes, &idx, &codes_s, &idx_s, &tmp, &child, &range, &parent, &box, &cost, &collapse,
+                          &flag, &nodes})
+            b->release();
+    }
+};
+
+// Depth-first renumbering of the reachable nodes: the left child follows its parent,
+// and interior nodes inside collapsed subtrees are dropped.
+void compact(const std::vector<BvhNodeHost>& in, BvhResult& res) {
+    res.nodes.clear();
+    res.max_depth = 0;
+    res.num_leaves = 0;
+    struct Item {
+        int32_t id, parent, side, depth;
+    };
+    std::vector<Item> st;
+    st.push_back({0, -1, 0, 1});
+    while (!st.empty()) {
+        const Item it = st.back();
+        st.pop_back();
+        const int32_t nid = (int32_t)res.nodes.size();
+        res.nodes.push_back(in[(size_t)it.id]);
+        if (it.parent >= 0) res.nodes[(size_t)it.parent].d[it.side] = nid;
+        res.max_depth = std::max(res.max_depth, it.depth);
+        const BvhNodeHost& h = in[(size_t)it.id];
+        for (int side = 1; side >= 0; side--) {   // right pushed first: left is visited next
+            if (h.d[side] < 0) res.num_leaves++;
+            else st.push_back({h.d[side], nid, side, it.depth + 1});
+        }
+    }
+}
+
+}  // namespace
+
+int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhResult& res) {
+    if (opts.max_leaf < 1 || opts.max_leaf > kMaxLeafTris) {
+        set_error("build_bvh_gpu: max_leaf %d outside [1, %d]", opts.max_leaf, kMaxLeafTris);
+        return SPK_ERR_INVALID;
+    }
+    if (n > (int64_t)1 << 28) {
+        set_error("build_bvh_gpu: %lld triangles (at most 2^28)", (long long)n);
+        return SPK_ERR_INVALID;
+    }
+    if (n < 2) {   // a one-triangle scene is one leaf: nothing to sort
+        res = build_bvh(verts, n, opts);
+        return SPK_OK;
+    }
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            const float c = (verts[9 * i + a] + verts[9 * i + 3 + a] + verts[9 * i + 6 + a]) * (1.0f / 3.0f);
+            lo[a] = std::min(lo[a], c);
+            hi[a] = std::max(hi[a], c);
+        }
+    float scale[3];
+    for (int a = 0; a < 3; a++) scale[a] = hi[a] > lo[a] ? 1024.0f / (hi[a] - lo[a]) : 0.0f;
+    const int N = (int)n;
+    Bufs B;
+    int rc;
+    if ((rc = B.dv.upload(verts, (size_t)n * 9 * 4)) || (rc = B.codes.alloc((size_t)n * 4)) ||
+        (rc = B.idx.alloc((size_t)n * 4)) || (rc = B.codes_s.alloc((size_t)n * 4)) || (rc = B.idx_s.alloc((size_t)n * 4)) ||
+        (rc = B.child.alloc((size_t)n * 8)) || (rc = B.range.alloc((size_t)n * 8)) ||
+        (rc = B.parent.alloc((size_t)(2 * n) * 4)) || (rc = B.box.alloc((size_t)(2 * n) * sizeof(BoxF))) ||
+        (rc = B.cost.alloc((size_t)(2 * n) * 4)) || (rc = B.collapse.alloc((size_t)n * 4)) ||
+        (rc = B.flag.alloc((size_t)n * 4)) || (rc = B.nodes.alloc((size_t)(n - 1) * sizeof(BvhNodeHost))))
+        return rc;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, 0, B.dv.as<float>(), (uint32_t)n,
+                       make_float3(lo[0], lo[1], lo[2]), make_float3(scale[0], scale[1], scale[2]), B.codes.as<uint32_t>(),
+                       B.idx.as<uint32_t>());
+    SPK_CHECK_HIP(hipGetLastError());
+    size_t tbytes = 0;
+    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, B.codes.as<uint32_t>(), B.codes_s.as<uint32_t>(),
+                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 30,
+                                                     (hipStream_t)0));
+    if ((rc = B.tmp.alloc(std::max<size_t>(tbytes, 16)))) return rc;
+    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, tbytes, B.codes.as<uint32_t>(), B.codes_s.as<uint32_t>(),
+                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 30,
+                                                     (hipStream_t)0));
+    hipLaunchKernelGGL(k_hierarchy, dim3(blocks), dim3(256), 0, 0, B.codes_s.as<uint32_t>(), N, B.child.as<int2>(),
+                       B.range.as<int2>(), B.parent.as<int>());
+    SPK_CHECK_HIP(hipGetLastError());
+    SPK_CHECK_HIP(hipMemset(B.flag.p, 0, (size_t)n * 4));
+    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, 0, B.dv.as<float>(), B.idx_s.as<uint32_t>(), N,
+                       B.child.as<int2>(), B.range.as<int2>(), B.parent.as<int>(), B.box.as<BoxF>(), B.cost.as<float>(),
+                       B.collapse.as<uint32_t>(), B.flag.as<uint32_t>(), opts.max_leaf, opts.node_cost, opts.tri_cost);
+    SPK_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(256), 0, 0, B.child.as<int2>(), B.range.as<int2>(), B.box.as<BoxF>(), N,
+                       B.collapse.as<uint32_t>(), B.nodes.as<BvhNodeHost>());
+    SPK_CHECK_HIP(hipGetLastError());
+    std::vector<BvhNodeHost> raw((size_t)(n - 1));
+    res.order.resize((size_t)n);
+    SPK_CHECK_HIP(hipMemcpy(raw.data(), B.nodes.p, (size_t)(n - 1) * sizeof(BvhNodeHost), hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(res.order.data(), B.idx_s.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    compact(raw, res);
+    return SPK_OK;
+}
+
+}  // namespace spk

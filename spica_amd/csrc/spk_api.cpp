@@ -293,10 +293,31 @@ int spk_init(int device) {
     return SPK_OK;
 }
 
-int spk_scene_create(const spk_scene_desc* d, spk_scene** out) {
+int spk_scene_create(const spk_scene_desc* d, spk_scene** out) { return spk_scene_create_ex(d, nullptr, out); }
+
+int spk_scene_build_opts_default(spk_scene_build_opts* o) {
+    clear_error();
+    if (!o) {
+        set_error("spk_scene_build_opts_default: null argument");
+        return SPK_ERR_INVALID;
+    }
+    o->builder = SPK_BVH_SAH;
+    o->max_leaf = 4;
+    return SPK_OK;
+}
+
+int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bopts, spk_scene** out) {
     clear_error();
     if (!d || !out) {
         set_error("spk_scene_create: null argument");
+        return SPK_ERR_INVALID;
+    }
+    spk_scene_build_opts bop;
+    spk_scene_build_opts_default(&bop);
+    if (bopts) bop = *bopts;
+    if ((bop.builder != SPK_BVH_SAH && bop.builder != SPK_BVH_LBVH) || bop.max_leaf < 1 || bop.max_leaf > kMaxLeafTris) {
+        set_error("spk_scene_create: bad build options (builder %d, max_leaf %d; max_leaf in [1, %d])", bop.builder,
+                  bop.max_leaf, kMaxLeafTris);
         return SPK_ERR_INVALID;
     }
     *out = nullptr;
@@ -354,7 +375,13 @@ int spk_scene_create(const spk_scene_desc* d, spk_scene** out) {
     // BVH
     auto t0 = std::chrono::steady_clock::now();
     BvhBuildOpts bo;
-    BvhResult bvh = build_bvh(d->verts, n, bo);
+    bo.max_leaf = bop.max_leaf;
+    BvhResult bvh;
+    if (bop.builder == SPK_BVH_LBVH) {
+        if ((rc = build_bvh_gpu(d->verts, n, bo, bvh))) return rc;
+    } else {
+        bvh = build_bvh(d->verts, n, bo);
+    }
     auto t1 = std::chrono::steady_clock::now();
 
     // Leaf-ordered triangle data (FP64 set-up, FP32 storage).

@@ -49,6 +49,8 @@ struct BvhResult {
 };
 
 BvhResult build_bvh(const float* verts, int64_t n, const BvhBuildOpts& opts);
+// Linear BVH on the device (Morton order, radix tree, bottom-up boxes); same node format.
+int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhResult& res);
 
 // ---- device buffers -----------------------------------------------------------
 struct DevBuf {

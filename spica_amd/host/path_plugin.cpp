@@ -105,32 +105,82 @@ Film::Film(int width, int height, const std::string& filename)
     : w_(width), h_(height), filename_(filename), rgb_((size_t)width * height * 3, 0.0) {
     if (width <= 0 || height <= 0) throw ParamError("Film: non-positive resolution");
 }
-Film::Film(RenderParams& p) : Film(p.getInt("width"), p.getInt("height"), p.getString("filename", std::string(""))) {}
+Film::Film(RenderParams& p)
+    : Film(p.getInt("width"), p.getInt("height"), p.getString("outputFile", std::string(""))) {}
 
 void Film::saveHDR(const std::string& path) const {
-    FILE* f = fopen(path.c_str(), "wb");
-    if (!f) throw std::runtime_error("Film::saveHDR: cannot open " + path);
-    fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", h_, w_);
-    std::vector<unsigned char> row((size_t)w_ * 4);
+    // Each scanline: 2, 2, width (big endian), then the R, G, B and exponent planes in
+    // literal runs of at most 127 bytes; a pixel is (c * m * 256 / max, e + 128) with
+    // max = m * 2^e, and black below 1e-32.
+    std::vector<unsigned char> out;
+    char head[160];
+    snprintf(head, sizeof(head),
+             "#?RADIANCE\n# written by spica_amd\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n-Y %d +X %d\n", h_,
+             w_);
+    out.insert(out.end(), head, head + strlen(head));
+    std::vector<unsigned char> px((size_t)w_ * 4);
     for (int y = 0; y < h_; y++) {
         for (int x = 0; x < w_; x++) {
             const double* c = &rgb_[((size_t)y * w_ + x) * 3];
-            double v = std::max(c[0], std::max(c[1], c[2]));
-            unsigned char* e = &row[(size_t)x * 4];
-            if (v < 1e-32) {
+            const double v = std::max(c[0], std::max(c[1], c[2]));
+            unsigned char* e = &px[(size_t)x * 4];
+            if (!(v > 1e-32)) {
                 e[0] = e[1] = e[2] = e[3] = 0;
             } else {
                 int ex;
-                double m = frexp(v, &ex) * 256.0 / v;
-                e[0] = (unsigned char)(c[0] * m);
-                e[1] = (unsigned char)(c[1] * m);
-                e[2] = (unsigned char)(c[2] * m);
+                const double m = frexp(v, &ex) * 256.0 / v;
+                for (int k = 0; k < 3; k++) e[k] = (unsigned char)std::min(255.0, std::max(0.0, c[k] * m));
                 e[3] = (unsigned char)(ex + 128);
             }
         }
-        fwrite(row.data(), 1, row.size(), f);
+        out.insert(out.end(), {2, 2, (unsigned char)((w_ >> 8) & 0xFF), (unsigned char)(w_ & 0xFF)});
+        for (int k = 0; k < 4; k++)
+            for (int x = 0; x < w_;) {
+                const int run = std::min(127, w_ - x);
+                out.push_back((unsigned char)run);
+                for (int j = x; j < x + run; j++) out.push_back(px[(size_t)j * 4 + k]);
+                x += run;
+            }
     }
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("Film::saveHDR: cannot open " + path);
+    const size_t wrote = fwrite(out.data(), 1, out.size(), f);
     fclose(f);
+    if (wrote != out.size()) throw std::runtime_error("Film::saveHDR: short write to " + path);
+}
+
+std::string Film::formatName(const std::string& fmt, int id) {
+    std::string out;
+    bool used = false;
+    for (size_t i = 0; i < fmt.size(); i++) {
+        if (fmt[i] != '%') {
+            out += fmt[i];
+            continue;
+        }
+        if (i + 1 < fmt.size() && fmt[i + 1] == '%') {
+            out += '%';
+            i++;
+            continue;
+        }
+        size_t j = i + 1;
+        while (j < fmt.size() && strchr("-+ 0#", fmt[j])) j++;
+        while (j < fmt.size() && fmt[j] >= '0' && fmt[j] <= '9') j++;
+        if (used || j >= fmt.size() || !strchr("diu", fmt[j]) || j - i > 8)
+            throw ParamError("Film: output name \"" + fmt + "\" needs at most one %d-style conversion");
+        char buf[64];
+        const std::string spec = fmt.substr(i, j - i + 1);
+        snprintf(buf, sizeof(buf), spec.c_str(), id);
+        out += buf;
+        used = true;
+        i = j;
+    }
+    return out;
+}
+
+std::string Film::save(int id) const {
+    const std::string path = formatName(filename_, id) + ".hdr";
+    saveHDR(path);
+    return path;
 }
 
 PerspectiveCamera::PerspectiveCamera(const Transform& toWorld, double fovDeg, const std::shared_ptr<Film>& film,
@@ -246,6 +296,7 @@ PathIntegrator::~PathIntegrator() = default;
 
 void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) {
     if (!camera) throw ParamError("PathIntegrator::render: no camera");
+    if (!camera->film()->filename().empty()) Film::formatName(camera->film()->filename(), 0);   // bad names fail first
     const int spp = params.getInt("sampleCount");
     const int maxDepth = params.getInt("maxDepth", 16);
     check(spk_init(params.getInt("device", 0)), "spk_init");
@@ -298,8 +349,15 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
     for (const auto& l : scene.lights())
         if (auto e = std::dynamic_pointer_cast<Envmap>(l)) e->fill(d);
 
+    // "accelerator" (sceneparser.cc:64, default "bvh"): spica's SAH tree, or "lbvh",
+    // a linear BVH built on the device.
+    spk_scene_build_opts bo;
+    check(spk_scene_build_opts_default(&bo), "spk_scene_build_opts_default");
+    const std::string accel = params.getString("accelerator", std::string("bvh"));
+    if (accel == "lbvh") bo.builder = SPK_BVH_LBVH;
+    else if (accel != "bvh") throw ParamError("PathIntegrator::render: unknown accelerator \"" + accel + "\"");
     spk_scene* s = nullptr;
-    check(spk_scene_create(&d, &s), "spk_scene_create");
+    check(spk_scene_create_ex(&d, &bo, &s), "spk_scene_create_ex");
     spk_camera_desc cd = camera->describe();
     spk_render_desc rd;
     memset(&rd, 0, sizeof(rd));
@@ -336,19 +394,36 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
             check(rc, "guiding");
         }
     }
-    rc = spk_render_run(r, spp, 1);
+    // spica saves after every pass (integrator.cc:100); "saveEvery" spaces the saves out.
+    const bool saving = !camera->film()->filename().empty();
+    const int every = std::max(1, std::min(spp, params.getInt("saveEvery", spp)));
     std::vector<double> sum((size_t)cd.width * cd.height * 3), cnt((size_t)cd.width * cd.height);
-    if (rc == SPK_OK) rc = spk_render_film(r, sum.data(), nullptr, cnt.data());
+    std::vector<double>& img = camera->film()->image();
+    std::string err;
+    for (int done = 0; done < spp && rc == SPK_OK && err.empty();) {
+        const int k = saving ? std::min(every, spp - done) : spp;
+        rc = spk_render_run(r, k, 1);
+        done += k;
+        if (rc == SPK_OK && (saving || done == spp)) {
+            rc = spk_render_film(r, sum.data(), nullptr, cnt.data());
+            // Film::save's division (film.cc:23-40)
+            for (size_t i = 0; rc == SPK_OK && i < cnt.size(); i++)
+                for (int c = 0; c < 3; c++) img[3 * i + c] = sum[3 * i + c] / (cnt[i] + 1e-12);
+            if (rc == SPK_OK && saving) {
+                try {
+                    camera->film()->save(done);
+                } catch (const std::exception& e) {
+                    err = e.what();
+                }
+            }
+        }
+    }
     if (rc == SPK_OK) rc = spk_render_stats_get(r, &stats_);
     spk_render_destroy(r);
     if (g) spk_guide_destroy(g);
     spk_scene_destroy(s);
     check(rc, "render");
-    // Film::save's division (film.cc:23-40)
-    std::vector<double>& img = camera->film()->image();
-    for (size_t i = 0; i < cnt.size(); i++)
-        for (int c = 0; c < 3; c++) img[3 * i + c] = sum[3 * i + c] / (cnt[i] + 1e-12);
-    if (!camera->film()->filename().empty()) camera->film()->saveHDR(camera->film()->filename());
+    if (!err.empty()) throw std::runtime_error(err);
 }
 
 }  // namespace spica

@@ -40,7 +40,7 @@ std::shared_ptr<SurfaceMaterial> material_from(const spk_material& m) {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s scene.bin out.f64 [plugin.so] [guiding]\n", argv[0]);
+        fprintf(stderr, "usage: %s scene.bin out.f64 [plugin.so] [guiding] [lbvh] [out=NAME] [every=K]\n", argv[0]);
         return 2;
     }
     std::string plugin = argc > 3 ? argv[3] : "libspica_path.so";
@@ -97,9 +97,16 @@ int main(int argc, char** argv) {
     params.add("maxDepth", (int)max_depth);
     if (sampler == SPK_SAMPLER_SOBOL) params.add("sampler", std::shared_ptr<CObject>(std::make_shared<Sobol>(seed)));
     else params.add("sampler", std::shared_ptr<CObject>(std::make_shared<Independent>(seed)));
-    if (argc > 4 && std::string(argv[4]) == "guiding") {
-        params.add("guiding", 1);
-        params.add("guideTrainSpp", 16);
+    for (int a = 4; a < argc; a++) {
+        const std::string opt = argv[a];
+        if (opt == "guiding") {
+            params.add("guiding", 1);
+            params.add("guideTrainSpp", 16);
+        } else if (opt == "lbvh") {
+            params.add("accelerator", std::string("lbvh"));
+        } else if (opt.rfind("every=", 0) == 0) {
+            params.add("saveEvery", atoi(opt.c_str() + 6));
+        }
     }
     std::shared_ptr<Integrator> integrator(dynamic_cast<Integrator*>(create(params)));
     printf("plugin: %s\n", describe());
@@ -122,7 +129,10 @@ int main(int argc, char** argv) {
     Scene scene(prims, lights);
     Transform toWorld;
     for (int k = 0; k < 16; k++) toWorld.m[k] = cam.to_world[k];
-    auto film = std::make_shared<Film>(cam.width, cam.height, std::string(""));
+    std::string out_name;
+    for (int a = 4; a < argc; a++)
+        if (std::string(argv[a]).rfind("out=", 0) == 0) out_name = argv[a] + 4;
+    auto film = std::make_shared<Film>(cam.width, cam.height, out_name);
     auto camera = std::make_shared<PerspectiveCamera>(toWorld, cam.fov_deg, film, cam.aperture_radius, cam.focus_distance);
 
     try {

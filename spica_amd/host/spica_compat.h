@@ -88,15 +88,20 @@ private:
 class Film : public CObject {
 public:
     Film(int width, int height, const std::string& filename);
-    explicit Film(RenderParams& params);  // "width", "height", "filename"
+    explicit Film(RenderParams& params);  // "width", "height", "outputFile" (hdrfilm.cc:15-19)
     int width() const { return w_; }
     int height() const { return h_; }
     // per film pixel mean RGB (row-major, x mirrored as spica's addPixel does)
     std::vector<double>& image() { return rgb_; }
     const std::vector<double>& image() const { return rgb_; }
     const std::string& filename() const { return filename_; }
-    // Radiance RGBE writer (the format spica's HDRFilm writes, core/image.cc:390-435)
+    // Radiance RGBE writer (the layout spica's HDRFilm writes, core/image.cc:390-435)
     void saveHDR(const std::string& path) const;
+    // Film::save(id) + HDRFilm::saveImage (film.cc:23-40, hdrfilm.cc:21-25): writes
+    // sprintf(filename, id) + ".hdr".  The name may hold one integer conversion
+    // (%d, %i, %u with flags and width) and %%; anything else raises ParamError.
+    std::string save(int id) const;
+    static std::string formatName(const std::string& fmt, int id);
 
 private:
     int w_, h_;
@@ -269,9 +274,12 @@ public:
     virtual void render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) = 0;
 };
 
-// Reads "sampleCount" (required), "maxDepth" (default 16), "sampler" (object) and the
-// GPU knobs "queueSize", "blockSize", "device".  Fills camera->film() with the
-// per-pixel mean; writes film->filename() as RGBE when it is non-empty.
+// Reads "sampleCount" (required), "maxDepth" (default 16), "sampler" (object),
+// "accelerator" ("bvh" or "lbvh") and the GPU knobs "queueSize", "blockSize", "device",
+// "guiding", "saveEvery".  Fills camera->film() with the per-pixel mean.  spica saves the
+// film after every sample pass as Film::save(samples so far) (integrator.cc:100); here it
+// is saved every "saveEvery" samples (default: once, after the last, which is the file
+// spica leaves behind) when the film has a non-empty name.
 class PathIntegrator : public Integrator {
 public:
     explicit PathIntegrator(const std::shared_ptr<Sampler>& sampler);

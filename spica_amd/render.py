@@ -27,12 +27,19 @@ def _ptr(t):
 
 
 class GpuScene:
-    def __init__(self, scene, device=0):
+    """builder: "sah" (host binned SAH, spica's BVH) or "lbvh" (linear BVH built on the device)."""
+
+    def __init__(self, scene, device=0, builder="sah", max_leaf=4):
         self.lib = init(device)
         self.scene = scene
         self._desc = scene.desc()
+        bo = _abi.SpkSceneBuildOpts()
+        _abi.check(self.lib.spk_scene_build_opts_default(ctypes.byref(bo)), "spk_scene_build_opts_default")
+        bo.builder = {"sah": _abi.SPK_BVH_SAH, "lbvh": _abi.SPK_BVH_LBVH}[builder]
+        bo.max_leaf = max_leaf
         h = ctypes.c_void_p()
-        _abi.check(self.lib.spk_scene_create(ctypes.byref(self._desc), ctypes.byref(h)), "spk_scene_create")
+        _abi.check(self.lib.spk_scene_create_ex(ctypes.byref(self._desc), ctypes.byref(bo), ctypes.byref(h)),
+                   "spk_scene_create_ex")
         self.handle = h
 
     def stats(self):

@@ -94,7 +94,7 @@ def write_rgbe(path, rgb):
     h, w, _ = rgb.shape
     v = rgb.max(axis=2)
     m, e = np.frexp(v)
-    ok = v >= 1e-32
+    ok = v > 1e-32
     sc = np.where(ok, m * 256.0 / np.where(ok, v, 1.0), 0.0)
     enc = np.zeros((h, w, 4), dtype=np.uint8)
     enc[..., :3] = np.clip(rgb * sc[..., None], 0, 255).astype(np.uint8)
@@ -402,27 +402,70 @@ def load_xml(path):
     return out
 
 
-def render_xml(path, out_hdr=None, queue_size=1 << 24, block_size=64, guiding=False):
-    """Render a scene file on the GPU and write the film as RGBE (the HDR film's save)."""
+def spica_filename(fmt, ident):
+    """sprintf(fmt, id) as Film::save names its file (film.cc:31-33), for names holding at
+    most one integer conversion (%d, %i, %u with flags and width) and %%."""
+    out, used, i = [], False, 0
+    while i < len(fmt):
+        ch = fmt[i]
+        if ch != "%":
+            out.append(ch)
+            i += 1
+            continue
+        if fmt[i + 1:i + 2] == "%":
+            out.append("%")
+            i += 2
+            continue
+        j = i + 1
+        while j < len(fmt) and fmt[j] in "-+ 0#":
+            j += 1
+        while j < len(fmt) and fmt[j].isdigit():
+            j += 1
+        if used or j >= len(fmt) or fmt[j] not in "diu" or j - i > 8:
+            raise ValueError(f"output name {fmt!r} needs at most one %d-style conversion")
+        out.append(("%" + fmt[i + 1:j] + "d") % ident)
+        used = True
+        i = j + 1
+    return "".join(out)
+
+
+def render_xml(path, output=None, save_every=None, queue_size=1 << 24, block_size=64, guiding=False):
+    """Render a scene file on the GPU and save the film as spica's HDR film does: the
+    file spica_filename(output, samples so far) + ".hdr" (hdrfilm.cc:21-25), every
+    save_every samples (spica saves after every pass, integrator.cc:100; default here:
+    once, after the last).  output defaults to the scene path without its extension, as
+    spica's command line sets "outputFile" (main.cc:37-42).  Returns (image, files)."""
     from . import render
     xs = load_xml(path)
     w = xs.workload()
+    if output is None:
+        output = xs.output or os.path.splitext(os.path.abspath(path))[0]
+    every = max(1, min(w.spp, save_every or w.spp))
+    spica_filename(output, 0)   # a bad name fails before the render
     gs = render.GpuScene(w.scene)
     gr = render.GpuRender(gs, w.camera, w.render_desc(), queue_size=queue_size, block_size=block_size)
     g = None
     if guiding:
         g = render.Guide()
         gr.set_guide(g)
-    gr.run(w.spp)
-    img = gr.image()
-    target = out_hdr or xs.output
-    if target:
-        write_rgbe(target, img)
-    gr.close()
-    if g is not None:
-        g.close()
-    gs.close()
-    return img
+    files = []
+    done = 0
+    img = None
+    try:
+        while done < w.spp:
+            k = min(every, w.spp - done)
+            gr.run(k)
+            done += k
+            img = gr.image()
+            name = spica_filename(output, done) + ".hdr"
+            write_rgbe(name, img)
+            files.append(name)
+    finally:
+        gr.close()
+        if g is not None:
+            g.close()
+        gs.close()
+    return img, files
 
 
 if __name__ == "__main__":
@@ -430,7 +473,9 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="render a spica scene file on the GPU")
     ap.add_argument("scene")
-    ap.add_argument("--out", default=None)
+    ap.add_argument("-o", "--output", default=None, help="output name (may hold a %%d for the sample count)")
+    ap.add_argument("--save-every", type=int, default=None, help="save every K samples (default: once)")
     ap.add_argument("--guiding", action="store_true")
     a = ap.parse_args()
-    render_xml(a.scene, a.out, guiding=a.guiding)
+    for f in render_xml(a.scene, a.output, a.save_every, guiding=a.guiding)[1]:
+        print(f)

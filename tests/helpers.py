@@ -88,8 +88,10 @@ def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6
     near_tie  both triangles hit (float64) at t within rel_t of each other;
     edge      the disagreement is explained by a hit within the float32 reach of a
               triangle edge, either side: barycentric distance below
-              max(edge, 32 * 2^-24 * (|o| + t) / sqrt(2 * area)), i.e. a few float32
-              rounding steps of the hit position measured in triangle sizes;
+              max(edge, 32 * 2^-24 * (|o| + t) / (sqrt(2 * area) * |cos|)), i.e. a few
+              float32 rounding steps of the hit position measured in triangle sizes,
+              divided by the cosine between ray and normal (the determinant of the
+              ray/triangle system shrinks with it, so grazing rays lose precision);
     other     anything else (a real error)."""
     idx = np.nonzero(p_gpu != p_cpu)[0]
     out = {"near_tie": 0, "edge": 0, "other": 0, "other_idx": []}
@@ -100,8 +102,11 @@ def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6
     v = verts.astype(np.float64)
     def reach(k, t):
         tri = v[k]
-        area2 = np.linalg.norm(np.cross(tri[1] - tri[0], tri[2] - tri[0]))
-        return max(edge, 32.0 * 2.0 ** -24 * (np.linalg.norm(o[j]) + abs(t)) / np.sqrt(max(area2, 1e-30)))
+        nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        area2 = np.linalg.norm(nrm)
+        cos = abs(np.dot(nrm, d[j])) / max(area2 * np.linalg.norm(d[j]), 1e-300)
+        return max(edge, 32.0 * 2.0 ** -24 * (np.linalg.norm(o[j]) + abs(t)) /
+                   (np.sqrt(max(area2, 1e-30)) * max(cos, 1e-3)))
 
     for j, i in enumerate(idx):
         pg, pc = int(p_gpu[i]), int(p_cpu[i])

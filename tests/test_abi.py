@@ -53,3 +53,17 @@ def test_bad_descriptors_are_rejected():
     assert lib.spk_scene_create(None, ctypes.byref(h)) == _abi.SPK_ERR_INVALID
     assert lib.spk_render_run(None, 1, 0) == _abi.SPK_ERR_INVALID
     assert lib.spk_sampler_u32(7, 0, 0, None, None, None, None, 0) == _abi.SPK_ERR_INVALID
+
+
+def test_scene_build_options_are_checked():
+    lib = _abi.load()
+    bo = _abi.SpkSceneBuildOpts()
+    assert lib.spk_scene_build_opts_default(ctypes.byref(bo)) == _abi.SPK_OK
+    assert (bo.builder, bo.max_leaf) == (_abi.SPK_BVH_SAH, 4)
+    d = configs.cornell_box().scene.desc()
+    h = ctypes.c_void_p()
+    for builder, leaf in ((2, 4), (-1, 4), (_abi.SPK_BVH_LBVH, 0), (_abi.SPK_BVH_LBVH, 9), (_abi.SPK_BVH_SAH, 100)):
+        bo.builder, bo.max_leaf = builder, leaf
+        rc = lib.spk_scene_create_ex(ctypes.byref(d), ctypes.byref(bo), ctypes.byref(h))
+        assert rc == _abi.SPK_ERR_INVALID and b"build options" in lib.spk_last_error()
+    assert lib.spk_scene_build_opts_default(None) == _abi.SPK_ERR_INVALID

@@ -20,10 +20,10 @@ from tests.helpers import classify_mismatches, furnace_room, ray_set, small_mixe
 pytestmark = pytest.mark.gpu
 
 
-def _trace_parity(gpu, oracle, scene, org, dirv, max_mismatch_frac):
+def _trace_parity(gpu, oracle, scene, org, dirv, max_mismatch_frac, **build):
     import torch
 
-    gs = gpu.GpuScene(scene)
+    gs = gpu.GpuScene(scene, **build)
     prim, t = gs.trace(torch.from_numpy(org).cuda(), torch.from_numpy(dirv).cuda())
     pg, tg = prim.cpu().numpy(), t.cpu().numpy()
     pc, tc, _ = oracle.scene(scene).trace(org.astype(np.float64), dirv.astype(np.float64), threads=16)
@@ -92,6 +92,89 @@ def test_trace_parity_c1_10m(gpu, oracle):
     w = configs.c1_spheres()
     org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 10_000_000, seed=1)
     _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-4)
+
+
+@pytest.mark.parametrize("max_leaf", [1, 4, 8])
+def test_lbvh_trace_parity_cornell(gpu, oracle, max_leaf):
+    """The device-built linear BVH answers the checker's closest hits like the SAH tree."""
+    w = configs.cornell_box()
+    org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 300000, seed=4)
+    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-2, builder="lbvh", max_leaf=max_leaf)
+
+
+@pytest.mark.parametrize("mesh", ["bunny", "kitten"])
+def test_lbvh_trace_parity_meshes(gpu, oracle, mesh):
+    tris, _ = meshio.load_ply(os.path.join(GOLDEN, f"{mesh}.ply"))
+    s = Scene()
+    s.add_triangles(tris, diffuse((0.5, 0.5, 0.5)))
+    rng = np.random.default_rng(5)
+    lo, hi = s.bounds()
+    n = 1000000
+    org = rng.uniform(lo - 0.25 * (hi - lo), hi + 0.25 * (hi - lo), size=(n, 3)).astype(np.float32)
+    dirv = (rng.uniform(lo, hi, size=(n, 3)) - org).astype(np.float32)
+    _trace_parity(gpu, oracle, s, org, dirv, 1e-3, builder="lbvh")
+
+
+def test_lbvh_trace_parity_c1(gpu, oracle):
+    w = configs.c1_spheres()
+    org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 2_000_000, seed=6)
+    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-4, builder="lbvh")
+
+
+def test_lbvh_degenerate_scenes(gpu, oracle):
+    """Tiny scenes (2..9 triangles) and scenes whose centroids all share one Morton code
+    (copies of one triangle, a thin sliver) build and trace like the checker."""
+    import torch
+
+    rng = np.random.default_rng(7)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    cases = [rng.uniform(-1, 1, size=(n, 3, 3)).astype(np.float32) for n in (1, 2, 3, 5, 9)]
+    cases.append(np.repeat(base[None], 37, axis=0) + np.linspace(0, 1e-6, 37, dtype=np.float32)[:, None, None])
+    cases.append(np.repeat(base[None], 50, axis=0))
+    for tris in cases:
+        s = Scene()
+        s.add_triangles(tris, diffuse((0.5, 0.5, 0.5)))
+        n = 20000
+        org = np.tile(np.array([[0.2, 0.2, 3.0]], np.float32), (n, 1)) + rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+        dirv = (rng.uniform(-1, 1, size=(n, 3)) * 0.3 + [0, 0, -1]).astype(np.float32)
+        gs = gpu.GpuScene(s, builder="lbvh", max_leaf=4)
+        st = gs.stats()
+        assert st["num_tris"] == len(tris)
+        prim, t = gs.trace(torch.from_numpy(org).cuda(), torch.from_numpy(dirv).cuda())
+        pc, tc, _ = oracle.scene(s).trace(org.astype(np.float64), dirv.astype(np.float64), threads=4)
+        pg = prim.cpu().numpy()
+        tg = t.cpu().numpy()
+        cls = classify_mismatches(s.freeze()["verts"], org, dirv, pg, tg, pc, tc)
+        assert cls["other"] == 0, (len(tris), cls.get("other_detail", [])[:3])
+        hit = (pc >= 0) & (pg >= 0)
+        assert np.allclose(tg[hit], tc[hit], rtol=1e-5, atol=1e-6), len(tris)
+        gs.close()
+
+
+def test_lbvh_and_sah_agree(gpu):
+    """Both trees over the same scene give the same closest t for every ray, and the
+    same triangle wherever the closest t is not shared by two triangles."""
+    import torch
+
+    w = configs.c1_spheres(n_spheres=256)
+    rng = np.random.default_rng(8)
+    lo, hi = w.scene.bounds()
+    n = 2_000_000
+    o = torch.from_numpy(rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)).cuda()
+    d = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    res = {}
+    for b in ("sah", "lbvh"):
+        gs = gpu.GpuScene(w.scene, builder=b)
+        res[b] = gs.trace(o, d)
+        st = gs.stats()
+        print(b, {k: st[k] for k in ("num_nodes", "num_leaves", "max_depth", "build_ms")})
+        gs.close()
+    (pa, ta), (pb, tb) = res["sah"], res["lbvh"]
+    assert torch.equal(pa >= 0, pb >= 0)
+    assert torch.equal(ta[pa >= 0], tb[pb >= 0])
+    diff = (pa != pb).sum().item()
+    print(f"closest triangle differs for {diff} of {n} rays (equal t)")
+    assert diff <= n * 1e-4
 
 
 def test_any_hit_matches_closest(gpu):

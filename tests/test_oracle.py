@@ -10,7 +10,7 @@ import pytest
 from spica_amd import configs, meshio
 from spica_amd.scene import Camera, Scene, diffuse, look_at
 from tests.conftest import GOLDEN
-from tests.helpers import furnace_room, small_mixed_scene, unit_vectors
+from tests.helpers import classify_mismatches, furnace_room, small_mixed_scene, unit_vectors
 
 # ---------------------------------------------------------------------------
 # geometry known answers (spica tests/test_geometry.cc, tests/test_trimesh.cc)
@@ -78,6 +78,26 @@ def test_bvh_matches_brute_force(oracle, mesh):
 # ---------------------------------------------------------------------------
 # samplers
 # ---------------------------------------------------------------------------
+
+
+def test_mismatch_classifier():
+    """The closest-hit mismatch classes used by the GPU parity tests: a hit 1e-3 outside a
+    unit triangle is a float32 edge case only for a grazing ray (the solve's determinant
+    shrinks with the cosine); head-on it is an error."""
+    verts = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
+    target = np.array([0.5005, 0.5005, 0.0])   # u + v = 1.001
+    cases = []
+    for d in (np.array([1.0, 0.0, -0.002]), np.array([0.0, 0.0, -1.0])):
+        d = d / np.linalg.norm(d)
+        cases.append((target - 20.0 * d, d))
+    org = np.array([c[0] for c in cases], np.float32)
+    dirv = np.array([c[1] for c in cases], np.float32)
+    p_gpu, t_gpu = np.array([0, 0]), np.array([20.0, 20.0], np.float32)
+    p_cpu, t_cpu = np.array([-1, -1]), np.array([-1.0, -1.0])
+    cls = classify_mismatches(verts, org[:1], dirv[:1], p_gpu[:1], t_gpu[:1], p_cpu[:1], t_cpu[:1])
+    assert (cls["edge"], cls["other"]) == (1, 0)
+    cls = classify_mismatches(verts, org[1:], dirv[1:], p_gpu[1:], t_gpu[1:], p_cpu[1:], t_cpu[1:])
+    assert (cls["edge"], cls["other"]) == (0, 1)
 
 
 def test_mt19937_known_answers(oracle):

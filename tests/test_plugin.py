@@ -15,6 +15,9 @@ DEMO = build.DEMO
 
 
 def test_plugin_exports_spica_entry_points():
+    from spica_amd import _abi
+
+    _abi.load()   # binds libspk.so to torch's HIP runtime before the plugin pulls it in
     lib = ctypes.CDLL(PLUGIN)
     lib.getDescription.restype = ctypes.c_char_p
     assert b"Path tracing" in lib.getDescription()
@@ -35,17 +38,20 @@ def test_plugin_reports_errors_without_aborting(tmp_path):
 
 
 @pytest.mark.gpu
-def test_plugin_render_equals_c_abi_render(gpu, tmp_path):
-    """The C++ plugin path and the Python handles over the same C ABI give the same film, bit for bit."""
+@pytest.mark.parametrize("accel", ["bvh", "lbvh"])
+def test_plugin_render_equals_c_abi_render(gpu, tmp_path, accel):
+    """The C++ plugin path and the Python handles over the same C ABI give the same film,
+    bit for bit, with either accelerator."""
     w = configs.cornell_box(width=64, height=48)
     sf = tmp_path / "s.bin"
     w.scene.save_bin(sf, w.camera, 32, 8, 0, 0)
     out = tmp_path / "img.f64"
-    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN] + (["lbvh"] if accel == "lbvh" else []), capture_output=True,
+                       text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     img = np.fromfile(out).reshape(48, 64, 3)
-    gs = gpu.GpuScene(w.scene)
+    gs = gpu.GpuScene(w.scene, builder="sah" if accel == "bvh" else "lbvh")
     gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=32))
     gr.run(32)
     s, q, c = gr.film()
@@ -74,3 +80,34 @@ def test_plugin_guiding_parameter(gpu, tmp_path):
     s, q, c = gr.film()
     plain = s / (c[..., None] + 1e-12)
     assert abs(img.mean() / plain.mean() - 1) < 0.01
+
+
+@pytest.mark.gpu
+def test_plugin_saves_like_film_save(gpu, tmp_path):
+    """"saveEvery" = 4 with outputFile "img%02d": the plugin writes img04.hdr and img08.hdr
+    (Film::save(id) + ".hdr", film.cc:23-40, hdrfilm.cc:21-25), each byte-equal to the RGBE
+    encoding of the render stopped at that sample count; a name with a non-integer
+    conversion is refused before rendering."""
+    from spica_amd import sceneio
+
+    w = configs.cornell_box(width=40, height=30)
+    sf = tmp_path / "s.bin"
+    w.scene.save_bin(sf, w.camera, 8, 8, 0, 0)
+    out = tmp_path / "img.f64"
+    name = str(tmp_path / "img%02d")
+    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN, "out=" + name, "every=4"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    for k in (4, 8):
+        gs = gpu.GpuScene(w.scene)
+        gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=k))
+        gr.run(k)
+        s, q, c = gr.film()
+        gr.close()
+        gs.close()
+        sceneio.write_rgbe(tmp_path / f"want{k}.hdr", s / (c[..., None] + 1e-12))
+        assert (tmp_path / f"img{k:02d}.hdr").read_bytes() == (tmp_path / f"want{k}.hdr").read_bytes()
+    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN, "out=" + str(tmp_path / "bad%s")], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 4 and "conversion" in r.stderr

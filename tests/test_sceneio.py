@@ -108,6 +108,27 @@ def test_rgbe_known_answer_and_round_trip(tmp_path):
     assert np.all(np.abs(back - img) <= big / 128.0 + 1e-30)
 
 
+def test_rgbe_black_threshold(tmp_path):
+    # HDRPixel (image.cc:72-88): a pixel whose largest channel is <= 1e-32 is written as zero
+    img = np.array([[[1e-32, 0, 0], [2e-32, 0, 0]]], dtype=np.float64)
+    sceneio.write_rgbe(tmp_path / "t.hdr", img)
+    raw = (tmp_path / "t.hdr").read_bytes()
+    body = raw[raw.index(b"+X 2\n") + 5:]
+    m, e = math.frexp(2e-32)
+    assert list(body[4:7]) == [2, 0, int(m * 256)] and list(body[13:16]) == [2, 0, e + 128]
+
+
+def test_output_names_follow_film_save():
+    """Film::save names its file sprintf(outputFile, id) (film.cc:31-33)."""
+    assert sceneio.spica_filename("out", 7) == "out"
+    assert sceneio.spica_filename("img%03d", 7) == "img007"
+    assert sceneio.spica_filename("a%%b%d", 12) == "a%b12"
+    assert sceneio.spica_filename("x%-4dy", 3) == "x3   y"
+    for bad in ("%s", "%d%d", "%f", "%", "%123456789d"):
+        with pytest.raises(ValueError):
+            sceneio.spica_filename(bad, 1)
+
+
 def test_rgbe_reads_run_length_runs(tmp_path):
     # a scanline of 4 equal pixels written with runs (count > 128 = repeat)
     hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 4\n"
@@ -163,6 +184,24 @@ def test_scene_file_renders_like_the_programmatic_scene(gpu, tmp_path):
         gs.close()
     for a, b in zip(*films):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_progressive_saves(gpu, tmp_path):
+    """Saves every 4 of 8 samples as <name>%02d.hdr: each file holds exactly the film of
+    a render stopped at that sample count."""
+    p, w = _cornell_xml(tmp_path, spp=8)
+    img, files = sceneio.render_xml(p, output=str(tmp_path / "img%02d"), save_every=4)
+    assert [os.path.basename(f) for f in files] == ["img04.hdr", "img08.hdr"]
+    for k, f in zip((4, 8), files):
+        gs = gpu.GpuScene(w.scene)
+        gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=k, sampler=_abi.SPK_SAMPLER_INDEPENDENT, seed=0))
+        gr.run(k)
+        ref = tmp_path / f"ref{k}.hdr"
+        sceneio.write_rgbe(ref, gr.image())
+        gr.close()
+        gs.close()
+        assert open(f, "rb").read() == ref.read_bytes()
 
 
 def test_zero_normals_mark_flat_triangles(oracle):
