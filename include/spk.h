@@ -92,7 +92,9 @@ typedef struct spk_render_opts {
 
 /* Creates a render of `scene` through `camera`: film, path queues and stream.
  * Replaces the set-up half of SamplerIntegrator::render
- * (sources/core/integrator.cc:46-62) and Film::Film (sources/core/film.cc:8-21). */
+ * (sources/core/integrator.cc:46-62) and Film::Film (sources/core/film.cc:8-21);
+ * desc->integrator picks PathIntegrator::Li (integrators/path/path.cc:42-125) or
+ * DirectLightingIntegrator::Li (integrators/directlighting/directlighting.cc:21-57). */
 int spk_render_create(spk_scene* scene, const spk_camera_desc* camera, const spk_render_desc* desc,
                       const spk_render_opts* opts, spk_render** out);
 int spk_render_destroy(spk_render* render);

@@ -99,15 +99,25 @@ enum spk_sampler_type {
     SPK_SAMPLER_MT19937 = 2      /* CPU checker only: spica's Independent/MT19937 stream */
 };
 
+/* The estimator: spica's "path" plugin (integrators/path/path.cc:42-125) or its
+ * "directlighting" plugin (integrators/directlighting/directlighting.cc:21-57: emission
+ * at the first hit, one-light NEE + MIS at every hit, recursion only through
+ * specular reflection / transmission, core/integrator.cc:112-152). */
+enum spk_integrator_type {
+    SPK_INTEGRATOR_PATH = 0,
+    SPK_INTEGRATOR_DIRECT = 1
+};
+
 /* Render request: "sampleCount" and "maxDepth" as spica reads them, plus the
- * counter-based sampler key. */
+ * counter-based sampler key and the integrator plugin. */
 typedef struct spk_render_desc {
     int32_t spp;                 /* "sampleCount" */
     int32_t max_depth;           /* "maxDepth" (spica default 16) */
     int32_t sampler;             /* enum spk_sampler_type */
     uint32_t seed;
-    int32_t rr_depth;            /* Russian roulette when bounces > rr_depth (spica: 3) */
+    int32_t rr_depth;            /* Russian roulette when bounces > rr_depth (spica: 3; path only) */
     int32_t first_sample;        /* sample index of the first pass (resume / split) */
+    int32_t integrator;          /* enum spk_integrator_type (0 = path) */
 } spk_render_desc;
 
 #ifdef __cplusplus

@@ -28,7 +28,7 @@ REF_UNITS = [
     "core/filter", "core/cobject", "core/renderparams", "core/scene", "core/visibility_tester", "core/texture",
     "core/material", "core/medium", "core/phase", "core/bssrdf", "core/mipmap", "core/interpolation", "core/float",
     "core/exception", "core/uv", "core/constant", "core/tmo", "core/birateral", "core/quaternion",
-    "accelerators/bvh", "integrators/path/path", "samplers/independent", "bsdfs/diffuse", "bsdfs/conductor",
+    "accelerators/bvh", "integrators/path/path", "integrators/directlighting/directlighting", "samplers/independent", "bsdfs/diffuse", "bsdfs/conductor",
     "bsdfs/dielectric", "bsdfs/roughconductor", "lights/area", "cameras/perspective", "filters/box", "films/hdrfilm",
 ]
 

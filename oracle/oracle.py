@@ -56,6 +56,7 @@ class Oracle:
         L.orc_env_eval.argtypes = [_vp, ctypes.c_int64, _d, _d, _d]
         L.orc_env_sample.argtypes = [_vp, ctypes.c_int64, _d, _d, _d, _d]
         L.orc_li_mt.argtypes = [_vp, _d, _d, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _d]
+        L.orc_li_mt_direct.argtypes = [_vp, _d, _d, ctypes.c_uint32, ctypes.c_int, _d]
         L.orc_render.argtypes = [_vp, _vp, _vp, _i32, ctypes.c_int64, ctypes.c_int, _d, _d, _i64, _i64]
         self.L = L
 
@@ -198,6 +199,13 @@ class OracleScene:
         self.orc.L.orc_li_mt(self.h, _dp(o), _dp(d), seed, max_depth, rr_depth, _dp(L))
         return L
 
+    def li_mt_direct(self, org, dirv, seed, max_depth):
+        """DirectLightingIntegrator::Li with spica's MT19937 stream seeded `seed`."""
+        o, d = (np.ascontiguousarray(x, dtype=np.float64) for x in (org, dirv))
+        L = np.zeros(3)
+        self.orc.L.orc_li_mt_direct(self.h, _dp(o), _dp(d), seed, max_depth, _dp(L))
+        return L
+
     def __del__(self):
         try:
             self.orc.L.orc_scene_free(self.h)
@@ -217,6 +225,7 @@ class Reference:
         L.ref_scene_free.argtypes = [_vp]
         L.ref_trace.argtypes = [_vp, ctypes.c_int64, _d, _d, ctypes.c_int, _i32, _d]
         L.ref_li.argtypes = [_vp, ctypes.c_int64, _d, _d, _u32, ctypes.c_int, _d]
+        L.ref_li_direct.argtypes = [_vp, ctypes.c_int64, _d, _d, _u32, ctypes.c_int, _d]
         L.ref_mt.argtypes = [ctypes.c_uint32, ctypes.c_int64, _d]
         L.ref_bxdf_sample.restype = ctypes.c_int
         L.ref_bxdf_sample.argtypes = [_vp, _d, _d, _d, _d, _d]
@@ -267,13 +276,15 @@ class RefScene:
         self.ref.L.ref_trace(self.h, n, _dp(org), _dp(dirv), 1 if any_hit else 0, prim.ctypes.data_as(_i32), _dp(t))
         return prim, t
 
-    def li(self, org, dirv, seeds, max_depth):
+    def li(self, org, dirv, seeds, max_depth, direct=False):
+        """spica's PathIntegrator::Li (or DirectLightingIntegrator::Li) per ray."""
         org = np.ascontiguousarray(org, dtype=np.float64)
         dirv = np.ascontiguousarray(dirv, dtype=np.float64)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
         n = len(org)
         L = np.zeros((n, 3))
-        self.ref.L.ref_li(self.h, n, _dp(org), _dp(dirv), seeds.ctypes.data_as(_u32), max_depth, _dp(L))
+        fn = self.ref.L.ref_li_direct if direct else self.ref.L.ref_li
+        fn(self.h, n, _dp(org), _dp(dirv), seeds.ctypes.data_as(_u32), max_depth, _dp(L))
         return L
 
     def __del__(self):

@@ -57,6 +57,11 @@
 #include "integrators/path/path.h"
 #undef createInstance
 #undef getDescription
+#define createInstance createInstance_directlighting
+#define getDescription getDescription_directlighting
+#include "integrators/directlighting/directlighting.h"
+#undef createInstance
+#undef getDescription
 #define createInstance createInstance_area
 #define getDescription getDescription_area
 #include "lights/area.h"
@@ -109,6 +114,35 @@ public:
     }
 };
 
+class DirectHarness : public DirectLightingIntegrator {
+public:
+    explicit DirectHarness(const std::shared_ptr<Sampler>& s) : DirectLightingIntegrator(s) {}
+    Spectrum li(const Scene& scene, RenderParams& params, const Ray& ray, Sampler& sampler, MemoryArena& arena) const {
+        return Li(scene, params, ray, sampler, arena, 0);
+    }
+};
+
+// PathIntegrator::Li (integrator 0) or DirectLightingIntegrator::Li (integrator 1) with
+// spica's Independent(seed) sampler, one fresh sampler per path.
+template <class H>
+void li_loop(RefScene* rs, int64_t n, const double* org, const double* dir, const uint32_t* seeds, int max_depth,
+                    double* L_out) {
+    RenderParams& params = RenderParams::getInstance();
+    params.add("maxDepth", max_depth);
+    auto dummy = std::make_shared<Independent>(0u);
+    H h(dummy);
+    MemoryArena arena;
+    for (int64_t i = 0; i < n; i++) {
+        Independent sampler(seeds[i]);
+        Ray ray(Point3d(org[3 * i], org[3 * i + 1], org[3 * i + 2]), Vector3d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+        Spectrum L = h.li(*rs->scene, params, ray, sampler, arena);
+        L_out[3 * i] = L.red();
+        L_out[3 * i + 1] = L.green();
+        L_out[3 * i + 2] = L.blue();
+        arena.reset();
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -159,23 +193,13 @@ void ref_trace(void* p, int64_t n, const double* org, const double* dir, int any
     }
 }
 
-// PathIntegrator::Li with spica's Independent(seed) sampler, one fresh sampler per path.
 void ref_li(void* p, int64_t n, const double* org, const double* dir, const uint32_t* seeds, int max_depth, double* L_out) {
-    RefScene* rs = (RefScene*)p;
-    RenderParams& params = RenderParams::getInstance();
-    params.add("maxDepth", max_depth);
-    auto dummy = std::make_shared<Independent>(0u);
-    LiHarness h(dummy);
-    MemoryArena arena;
-    for (int64_t i = 0; i < n; i++) {
-        Independent sampler(seeds[i]);
-        Ray ray(Point3d(org[3 * i], org[3 * i + 1], org[3 * i + 2]), Vector3d(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
-        Spectrum L = h.li(*rs->scene, params, ray, sampler, arena);
-        L_out[3 * i] = L.red();
-        L_out[3 * i + 1] = L.green();
-        L_out[3 * i + 2] = L.blue();
-        arena.reset();
-    }
+    li_loop<LiHarness>((RefScene*)p, n, org, dir, seeds, max_depth, L_out);
+}
+
+void ref_li_direct(void* p, int64_t n, const double* org, const double* dir, const uint32_t* seeds, int max_depth,
+                   double* L_out) {
+    li_loop<DirectHarness>((RefScene*)p, n, org, dir, seeds, max_depth, L_out);
 }
 
 // MT19937 stream of spica's Random (core/random.cc:95-97).

@@ -1,7 +1,8 @@
 /*
  * spk_oracle.c -- CPU checker for the MI355X path tracer.  TEST INFRASTRUCTURE ONLY.
  *
- * A plain-C, double-precision restatement of spica's PathIntegrator path: the
+ * A plain-C, double-precision restatement of spica's PathIntegrator path (and of the
+ * DirectLightingIntegrator next to it): the
  * estimator, constants, sampling order, tie rules and edge rules of the reference,
  * written from its sources (each function cites the file:line it follows, paths
  * relative to spica's `sources/`).  Only the tests, the smoke check and the CPU
@@ -1541,6 +1542,62 @@ static V3 Li_path(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int rrDept
     return L;
 }
 
+/* DirectLightingIntegrator::Li (integrators/directlighting/directlighting.cc:21-57) with
+ * SamplerIntegrator::specularReflect / specularTransmit (core/integrator.cc:112-152).
+ * `depth` is spica's argument: the recursion passes depth + 1 into specularReflect, which
+ * calls Li with its own depth + 1, so each specular bounce adds 2.  Draw order per call
+ * (MT19937): light pick, light uv, BSDF uv (uniformSampleOneLight, always drawn when the
+ * scene has lights), then while depth + 1 < maxDepth the reflection uv (and its whole
+ * subtree), then the transmission uv.  Counter-based dimensions: a block of 10 per
+ * recursion level (pick 0, light 1-2, BSDF 3-4, reflection 5-6, transmission 7-8); the
+ * supported lobes continue through at most one of the two specular calls (a
+ * FresnelSpecular lobe, type R|T|S, is no subset of R|S or T|S, so BSDF::sample finds no
+ * component for either, bsdf.cc:28-36,65-73), so one block per level suffices. */
+static V3 Li_direct(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int depth, int level, Counts* cp) {
+    const uint32_t base = 4u + 10u * (uint32_t)level;
+    int prim = -1;
+    double th = 0, u = 0, v = 0;
+    if (!scene_closest(s, &ray, &prim, &th, &u, &v, cp)) {
+        /* sum of Le(ray) over the lights: only the environment map emits along a ray */
+        return s->env.present ? env_Le(&s->env, ray.d) : v3(0, 0, 0);
+    }
+    Isect is;
+    make_isect(&s->tris[prim], &ray, th, u, v, prim, &is);
+    BSDF b;
+    setup_bsdf(&s->mats[s->mat[prim]], &is, &b);
+    V3 L = v3(0, 0, 0);
+    if (depth == 0) {
+        int li = s->light[prim];
+        if (li >= 0 && vdot(is.n, vneg(ray.d)) > 0.0) L = vadd(L, s->area_L[li]);
+    }
+    if (s->nlights > 0) {
+        double upick = smp(sp, base + 0);
+        int li = (int)(upick * s->nlights);
+        if (li > s->nlights - 1) li = s->nlights - 1;
+        double l0 = smp(sp, base + 1), l1 = smp(sp, base + 2);
+        double s0 = smp(sp, base + 3), s1 = smp(sp, base + 4);
+        V3 Ld = estimate_direct(s, &is, &b, li, l0, l1, s0, s1, cp);
+        L = vadd(L, vmul(Ld, (double)s->nlights));
+    }
+    if (depth + 1 < maxDepth) {
+        const int masks[2] = {BX_REFLECTION | BX_SPECULAR, BX_TRANSMISSION | BX_SPECULAR};
+        for (int c = 0; c < 2; c++) {
+            double u0 = smp(sp, base + 5 + 2 * c), u1 = smp(sp, base + 6 + 2 * c);
+            V3 wi = v3(0, 0, 0);
+            double pdf = 0.0;
+            int st = 0;
+            V3 f = bsdf_sample(&b, is.wo, &wi, u0, u1, &pdf, masks[c], &st);
+            double cosg = vabsdot(wi, is.n);
+            if (pdf > 0.0 && !vis_black(f) && cosg != 0.0) {
+                Ray r = make_ray(offset_ray_origin(is.pos, is.n, wi), wi, INFTY);
+                V3 Lr = Li_direct(s, r, sp, maxDepth, depth + 2, level + 1, cp);
+                L = vadd(L, vmul(vcmul(f, Lr), cosg / pdf));
+            }
+        }
+    }
+    return L;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Camera (cameras/perspective.cc:32-74, core/camera.cc:20-38,               */
 /* core/transform.cc:60-76, 205-212)                                          */
@@ -1910,6 +1967,20 @@ EXPORT void orc_li_mt(void* p, const double* org, const double* dir, uint32_t se
     L_out[2] = L.z;
 }
 
+/* The same for DirectLightingIntegrator::Li (directlighting.cc:21-57). */
+EXPORT void orc_li_mt_direct(void* p, const double* org, const double* dir, uint32_t seed, int max_depth,
+                             double* L_out) {
+    Scene* s = (Scene*)p;
+    MT mt;
+    mt_seed(&mt, seed);
+    Sampler sp = {SPK_SAMPLER_MT19937, seed, 0, 0, &mt};
+    Ray r = make_ray(v3(org[0], org[1], org[2]), v3(dir[0], dir[1], dir[2]), INFTY);
+    V3 L = Li_direct(s, r, &sp, max_depth, 0, 0, NULL);
+    L_out[0] = L.x;
+    L_out[1] = L.y;
+    L_out[2] = L.z;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Render (integrator.cc:46-110, film.cc:23-40, 65-74)                       */
 /* ------------------------------------------------------------------------- */
@@ -1945,7 +2016,18 @@ static void render_pixel(RenderJob* j, int64_t k, PathStats* ps) {
         double f0 = smp(&sp, 0), f1 = smp(&sp, 1);
         double l0 = smp(&sp, 2), l1 = smp(&sp, 3);
         Ray r = cam_ray(c, x, y, f0, f1, l0, l1);
-        V3 L = Li_path(j->s, r, &sp, j->rd.max_depth, j->rd.rr_depth, ps);
+        V3 L;
+        if (j->rd.integrator == SPK_INTEGRATOR_DIRECT) {
+            Counts cnt = {0, 0, 0};
+            L = Li_direct(j->s, r, &sp, j->rd.max_depth, 0, 0, ps ? &cnt : NULL);
+            if (ps) {
+                ps->rays += cnt.rays;
+                ps->nodes += cnt.nodes;
+                ps->tris += cnt.tris;
+            }
+        } else {
+            L = Li_path(j->s, r, &sp, j->rd.max_depth, j->rd.rr_depth, ps);
+        }
         j->sum[3 * k] += L.x;
         j->sum[3 * k + 1] += L.y;
         j->sum[3 * k + 2] += L.z;

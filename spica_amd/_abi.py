@@ -18,6 +18,9 @@ SPK_SAMPLER_INDEPENDENT = 0
 SPK_SAMPLER_SOBOL = 1
 SPK_SAMPLER_MT19937 = 2
 
+SPK_INTEGRATOR_PATH = 0
+SPK_INTEGRATOR_DIRECT = 1
+
 SPK_BVH_SAH = 0
 SPK_BVH_LBVH = 1
 
@@ -87,6 +90,7 @@ class SpkRenderDesc(ctypes.Structure):
         ("seed", ctypes.c_uint32),
         ("rr_depth", ctypes.c_int32),
         ("first_sample", ctypes.c_int32),
+        ("integrator", ctypes.c_int32),
     ]
 
 

@@ -35,14 +35,19 @@ def build_lib(verbose=False):
 
 HOST = os.path.join(HERE, "host")
 PLUGIN = os.path.join(HERE, "libspica_path.so")
+PLUGIN_DIRECT = os.path.join(HERE, "libspica_directlighting.so")
 DEMO = os.path.join(HOST, "plugin_demo")
 
 
 def build_plugin(verbose=False):
-    """The spica-compatible "path" plugin (C++ over the C ABI) and its demo driver."""
+    """The spica-compatible "path" and "directlighting" plugins (C++ over the C ABI) and
+    the demo driver."""
     cmds = [
         ["g++", "-std=c++17", "-O2", "-Wall", "-fPIC", "-shared", os.path.join(HOST, "path_plugin.cpp"),
          "-L" + HERE, "-lspk", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", PLUGIN + ".tmp"],
+        ["g++", "-std=c++17", "-O2", "-Wall", "-fPIC", "-shared", "-DSPK_PLUGIN_DIRECTLIGHTING",
+         os.path.join(HOST, "path_plugin.cpp"), "-L" + HERE, "-lspk", "-Wl,-rpath,$ORIGIN",
+         "-Wl,-rpath-link,/opt/rocm/lib", "-o", PLUGIN_DIRECT + ".tmp"],
         ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HOST, "plugin_demo.cpp"), "-L" + HERE, "-lspica_path",
          "-lspk", "-ldl", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib", "-o", DEMO + ".tmp"],
     ]
@@ -55,6 +60,8 @@ def build_plugin(verbose=False):
             raise RuntimeError("g++ failed building the path plugin")
         if i == 0:
             os.replace(PLUGIN + ".tmp", PLUGIN)
+        if i == 1:
+            os.replace(PLUGIN_DIRECT + ".tmp", PLUGIN_DIRECT)
     os.replace(DEMO + ".tmp", DEMO)
     return PLUGIN
 

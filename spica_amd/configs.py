@@ -28,8 +28,9 @@ from .scene import GOLD, SILVER, Camera, Scene, box, conductor, dielectric, diff
 
 
 class Workload:
-    def __init__(self, name, scene, camera, spp, max_depth, sampler, seed, rr_depth=3):
+    def __init__(self, name, scene, camera, spp, max_depth, sampler, seed, rr_depth=3, integrator=_abi.SPK_INTEGRATOR_PATH):
         self.name = name
+        self.integrator = integrator
         self.scene = scene
         self.camera = camera
         self.spp = spp
@@ -38,8 +39,9 @@ class Workload:
         self.seed = seed
         self.rr_depth = rr_depth
 
-    def render_desc(self, spp=None, first_sample=0, sampler=None, seed=None, max_depth=None):
+    def render_desc(self, spp=None, first_sample=0, sampler=None, seed=None, max_depth=None, integrator=None):
         d = _abi.SpkRenderDesc()
+        d.integrator = self.integrator if integrator is None else integrator
         d.spp = self.spp if spp is None else spp
         d.max_depth = self.max_depth if max_depth is None else max_depth
         d.sampler = self.sampler if sampler is None else sampler

@@ -681,6 +681,10 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
         set_error("spk_render_create: sampler %d not available on the GPU", desc->sampler);
         return SPK_ERR_INVALID;
     }
+    if (desc->integrator != SPK_INTEGRATOR_PATH && desc->integrator != SPK_INTEGRATOR_DIRECT) {
+        set_error("spk_render_create: unknown integrator %d", desc->integrator);
+        return SPK_ERR_INVALID;
+    }
     if (cam->to_world[12] != 0.0 || cam->to_world[13] != 0.0 || cam->to_world[14] != 0.0 || cam->to_world[15] != 1.0) {
         set_error("spk_render_create: camera toWorld must be affine");
         return SPK_ERR_INVALID;
@@ -726,6 +730,7 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     c.rr_depth = desc->rr_depth;
     c.sampler = desc->sampler;
     c.seed = desc->seed;
+    c.integrator = desc->integrator;
 
     // Buffers.
     int rc;
@@ -952,6 +957,10 @@ int spk_render_set_guide(spk_render* r, spk_guide* g) {
     SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
     if (g && g->device != r->device) {
         set_error("spk_render_set_guide: guide and render on different devices");
+        return SPK_ERR_INVALID;
+    }
+    if (g && r->cfg.integrator != SPK_INTEGRATOR_PATH) {
+        set_error("spk_render_set_guide: the learned proposal samples path continuations; directlighting has none");
         return SPK_ERR_INVALID;
     }
     if (g) SPK_CHECK_HIP(hipStreamSynchronize(g->stream));

@@ -115,6 +115,7 @@ struct ConfigDev {
     int rr_depth;
     int sampler;
     uint32_t seed;
+    int integrator;         // SPK_INTEGRATOR_PATH / SPK_INTEGRATOR_DIRECT
 };
 
 // Device-side per-pass state: queue lengths and work counters per bounce.

@@ -1109,9 +1109,12 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
     const int kind = cfgp->sampler;
     const uint32_t cfg_seed = cfgp->seed;
     const int max_depth = cfgp->max_depth, rr_depth = cfgp->rr_depth;
+    // directlighting.cc:21-57: emission at the first hit only, NEE at every hit, and the
+    // recursion only through specular reflection (see the continuation below)
+    const bool direct = cfgp->integrator == SPK_INTEGRATOR_DIRECT;
     const uint32_t sample_base = ps->sample_base;
     const int cur = depth & 1, nxt = cur ^ 1;
-    const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
+    const uint32_t dim_base = direct ? 4u + 10u * (uint32_t)depth : 4u + 8u * (uint32_t)depth;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < count;
@@ -1140,9 +1143,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
             const uint32_t sample = BATCH ? (flags >> 7) : sample_base + (flags >> 7);
             float3 ro = xyz(O), rd = xyz(D), beta = xyz(B);
             const int32_t k = __float_as_int(H.w);
-            const bool emit = depth == 0 || specular;
+            const bool emit = direct ? depth == 0 : (depth == 0 || specular);
             if (k < 0) {
-                if (emit && sc.env.present) Ladd = cmul(beta, env_Le(sc.env, rd));
+                // directlighting adds the lights' Le(ray) at every miss (directlighting.cc:30-34)
+                if ((emit || direct) && sc.env.present) Ladd = cmul(beta, env_Le(sc.env, rd));
             } else {
                 float4 NG = sc.tri_ng[k];
                 int2 ids = sc.tri_ids[k];
@@ -1150,7 +1154,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                 uint32_t mbits = __float_as_uint(NG.w);
                 float3 p = add(ro, mul(rd, H.x));
                 if (emit && ids.y >= 0 && dot(ng, neg(rd)) > 0.0f) Ladd = cmul(beta, xyz(sc.lights[ids.y].L));
-                if (depth < max_depth) {
+                if (direct || depth < max_depth) {
                     // BSDF frame (bsdf.cc:14-21 from triangle.cc:127-155)
                     Bsdf b;
                     bsdf_material(sc, mbits, b);
@@ -1226,7 +1230,28 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                     }
                     // Continuation (path.cc:83-94) and Russian roulette (path.cc:116-121); with a
                     // guide the non-specular vertices go to the guide queue instead.
-                    if (GUIDED && guide_eligible(b)) {
+                    if (direct) {
+                        // specularReflect (integrator.cc:112-130) while depth + 1 < maxDepth, where
+                        // spica's depth grows by 2 per specular bounce (Li -> specularReflect(depth + 1)
+                        // -> Li(depth + 2)); weighted by |wi . n_geometric| / pdf, no roulette.
+                        // specularTransmit never finds a component here: FresnelSpecular (R|T|S) is
+                        // no subset of T|S, and no other lobe transmits.
+                        if (2 * depth + 1 < max_depth) {
+                            float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
+                            float pdf = 0.0f;
+                            int st = 0;
+                            float3 wi = mk(0, 0, 1);
+                            float3 f = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf, BX_REFLECTION | BX_SPECULAR, st);
+                            const float cg = fabsf(dot(wi, ng));
+                            if (pdf > 0.0f && !black(f) && cg != 0.0f) {
+                                pushA = true;
+                                nA_o = offset_origin(p, ng, wi);
+                                nA_d = wi;
+                                nA_beta = cmul(beta, mul(f, cg / pdf));
+                                nA_flags = (flags & ~0x7fu) | (uint32_t)(depth + 1) | (1u << 6);
+                            }
+                        }
+                    } else if (GUIDED && guide_eligible(b)) {
                         pushG = true;
                         gq0 = make_float4(p.x, p.y, p.z, __uint_as_float(path));
                         gq1 = make_float4(wo.x, wo.y, wo.z, __uint_as_float(flags));

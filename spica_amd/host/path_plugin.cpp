@@ -1,6 +1,8 @@
-// path_plugin.cpp -- the "path" integrator plugin on the MI355X (libspica_path.so).
+// path_plugin.cpp -- the "path" integrator plugin on the MI355X (libspica_path.so); built
+// with SPK_PLUGIN_DIRECTLIGHTING it is the "directlighting" plugin
+// (libspica_directlighting.so).
 //
-// PathIntegrator::render replaces SamplerIntegrator::render + PathIntegrator::Li
+// SamplerIntegrator::render replaces SamplerIntegrator::render + PathIntegrator::Li
 // (sources/core/integrator.cc:46-110, sources/integrators/path/path.cc:42-125): it
 // flattens the scene into spk_scene_desc, the camera into spk_camera_desc and the
 // parameters into spk_render_desc, then calls the C ABI of libspk.so.
@@ -287,15 +289,23 @@ void check(int rc, const char* what) {
 }
 }  // namespace
 
-PathIntegrator::PathIntegrator(const std::shared_ptr<Sampler>& sampler) : sampler_(sampler) {
+SamplerIntegrator::SamplerIntegrator(const std::shared_ptr<Sampler>& sampler, int integrator)
+    : sampler_(sampler), integrator_(integrator) {
     if (!sampler_) sampler_ = std::make_shared<Independent>(0u);
 }
+SamplerIntegrator::~SamplerIntegrator() = default;
+
+PathIntegrator::PathIntegrator(const std::shared_ptr<Sampler>& sampler) : SamplerIntegrator(sampler, SPK_INTEGRATOR_PATH) {}
 PathIntegrator::PathIntegrator(RenderParams& p)
     : PathIntegrator(std::dynamic_pointer_cast<Sampler>(p.getObject("sampler", nullptr))) {}
-PathIntegrator::~PathIntegrator() = default;
 
-void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) {
-    if (!camera) throw ParamError("PathIntegrator::render: no camera");
+DirectLightingIntegrator::DirectLightingIntegrator(const std::shared_ptr<Sampler>& sampler)
+    : SamplerIntegrator(sampler, SPK_INTEGRATOR_DIRECT) {}
+DirectLightingIntegrator::DirectLightingIntegrator(RenderParams& p)
+    : DirectLightingIntegrator(std::dynamic_pointer_cast<Sampler>(p.getObject("sampler", nullptr))) {}
+
+void SamplerIntegrator::render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) {
+    if (!camera) throw ParamError("SamplerIntegrator::render: no camera");
     if (!camera->film()->filename().empty()) Film::formatName(camera->film()->filename(), 0);   // bad names fail first
     const int spp = params.getInt("sampleCount");
     const int maxDepth = params.getInt("maxDepth", 16);
@@ -311,7 +321,7 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
     std::vector<double> area_rad;
     for (size_t i = 0; i < prims.size(); i++) {
         const auto& p = prims[i];
-        if (!p->shape() || !p->material()) throw ParamError("PathIntegrator::render: primitive without shape or BSDF");
+        if (!p->shape() || !p->material()) throw ParamError("SamplerIntegrator::render: primitive without shape or BSDF");
         for (int v = 0; v < 3; v++) {
             const Point3d& q = (*p->shape())[v];
             verts[9 * i + 3 * v] = (float)q.x;
@@ -355,7 +365,7 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
     check(spk_scene_build_opts_default(&bo), "spk_scene_build_opts_default");
     const std::string accel = params.getString("accelerator", std::string("bvh"));
     if (accel == "lbvh") bo.builder = SPK_BVH_LBVH;
-    else if (accel != "bvh") throw ParamError("PathIntegrator::render: unknown accelerator \"" + accel + "\"");
+    else if (accel != "bvh") throw ParamError("SamplerIntegrator::render: unknown accelerator \"" + accel + "\"");
     spk_scene* s = nullptr;
     check(spk_scene_create_ex(&d, &bo, &s), "spk_scene_create_ex");
     spk_camera_desc cd = camera->describe();
@@ -367,6 +377,7 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
     rd.seed = sampler_->seed();
     rd.rr_depth = 3;
     rd.first_sample = 0;
+    rd.integrator = integrator_;
     spk_render_opts o;
     memset(&o, 0, sizeof(o));
     o.queue_size = params.getInt("queueSize", 1 << 24);
@@ -429,6 +440,11 @@ void PathIntegrator::render(const std::shared_ptr<const Camera>& camera, const S
 }  // namespace spica
 
 extern "C" {
+#ifdef SPK_PLUGIN_DIRECTLIGHTING
+spica::CObject* createInstance(spica::RenderParams& params) { return new spica::DirectLightingIntegrator(params); }
+const char* getDescription() { return "Integrate only direct lighting (MI355X wavefront)"; }
+#else
 spica::CObject* createInstance(spica::RenderParams& params) { return new spica::PathIntegrator(params); }
 const char* getDescription() { return "Path tracing integrator (MI355X wavefront)"; }
+#endif
 }

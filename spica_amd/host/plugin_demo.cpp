@@ -141,7 +141,7 @@ int main(int argc, char** argv) {
         fprintf(stderr, "render failed: %s\n", e.what());
         return 4;
     }
-    const auto* pi = dynamic_cast<PathIntegrator*>(integrator.get());
+    const auto* pi = dynamic_cast<SamplerIntegrator*>(integrator.get());
     if (pi) {
         const spk_render_stats& st = pi->lastStats();
         printf("paths %lld rays extend %lld shadow %lld mis %lld guided %lld\n", (long long)st.paths,

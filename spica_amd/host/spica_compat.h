@@ -4,6 +4,7 @@
 // caller written against spica drives the MI355X path tracer the same way:
 //   PluginManager::initModule("path")  -> dlopen(".../libspica_path.so")   (core/cobject.cc:12,32-40)
 //   createInstance(RenderParams&)      -> new PathIntegrator(params)        (core/cobject.h:56-64, path.h:41)
+//   ("directlighting": libspica_directlighting.so -> DirectLightingIntegrator, directlighting.h:28)
 //   integrator->render(camera, scene, params)                               (core/integrator.h:34-36)
 // Scene objects (Triangle, GeometricPrimitive, Diffuse, Conductor, Dielectric,
 // RoughConductor, AreaLight, Envmap, PerspectiveCamera, HDRFilm) take spica's
@@ -274,24 +275,41 @@ public:
     virtual void render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) = 0;
 };
 
-// Reads "sampleCount" (required), "maxDepth" (default 16), "sampler" (object),
-// "accelerator" ("bvh" or "lbvh") and the GPU knobs "queueSize", "blockSize", "device",
-// "guiding", "saveEvery".  Fills camera->film() with the per-pixel mean.  spica saves the
-// film after every sample pass as Film::save(samples so far) (integrator.cc:100); here it
-// is saved every "saveEvery" samples (default: once, after the last, which is the file
-// spica leaves behind) when the film has a non-empty name.
-class PathIntegrator : public Integrator {
+// SamplerIntegrator::render (core/integrator.cc:46-110) on the GPU, for the estimator a
+// subclass names.  Reads "sampleCount" (required), "maxDepth" (default 16), "sampler"
+// (object), "accelerator" ("bvh" or "lbvh") and the GPU knobs "queueSize", "blockSize",
+// "device", "guiding" (path only), "saveEvery".  Fills camera->film() with the per-pixel
+// mean.  spica saves the film after every sample pass as Film::save(samples so far)
+// (integrator.cc:100); here it is saved every "saveEvery" samples (default: once, after
+// the last, which is the file spica leaves behind) when the film has a non-empty name.
+class SamplerIntegrator : public Integrator {
 public:
-    explicit PathIntegrator(const std::shared_ptr<Sampler>& sampler);
-    explicit PathIntegrator(RenderParams& params);
-    ~PathIntegrator() override;
+    ~SamplerIntegrator() override;
     void render(const std::shared_ptr<const Camera>& camera, const Scene& scene, RenderParams& params) override;
     // traced-ray counters of the last render (extend, shadow, MIS)
     const spk_render_stats& lastStats() const { return stats_; }
 
+protected:
+    SamplerIntegrator(const std::shared_ptr<Sampler>& sampler, int integrator);
+
 private:
     std::shared_ptr<Sampler> sampler_;
+    int integrator_;
     spk_render_stats stats_{};
+};
+
+// PathIntegrator::Li (integrators/path/path.cc:42-125).
+class PathIntegrator : public SamplerIntegrator {
+public:
+    explicit PathIntegrator(const std::shared_ptr<Sampler>& sampler);
+    explicit PathIntegrator(RenderParams& params);
+};
+
+// DirectLightingIntegrator::Li (integrators/directlighting/directlighting.cc:21-57).
+class DirectLightingIntegrator : public SamplerIntegrator {
+public:
+    explicit DirectLightingIntegrator(const std::shared_ptr<Sampler>& sampler);
+    explicit DirectLightingIntegrator(RenderParams& params);
 };
 
 }  // namespace spica

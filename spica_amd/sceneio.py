@@ -21,7 +21,8 @@ render settings, following the parser's rules:
 * vectors and colours parse as "x y z", "x, y, z" or one value for all three
   (vector3d_detail.h:30-44).
 
-Supported plugins are the ones on the path tracer's path: integrator "path"; sensor
+Supported plugins are the ones on the path tracer's path: integrators "path" and
+"directlighting"; sensor
 "perspective"; film "hdrfilm"; samplers "independent" (and "sobol", "halton",
 "ldsampler", which map to the counter-based Sobol stream); bsdfs "diffuse",
 "conductor", "dielectric", "roughconductor" with distribution "ggx"; shapes "ply" and
@@ -236,8 +237,9 @@ class XmlScene:
 
     def workload(self):
         from .configs import Workload
+        kind = _abi.SPK_INTEGRATOR_DIRECT if self.integrator == "directlighting" else _abi.SPK_INTEGRATOR_PATH
         return Workload(os.path.basename(self.output or "scene"), self.scene, self.camera, self.spp, self.max_depth,
-                        self.sampler, self.seed)
+                        self.sampler, self.seed, integrator=kind)
 
 
 def _material(kind, p):
@@ -321,8 +323,8 @@ def load_xml(path):
                 p.add("bsdf", obj)
         elif tag == "integrator":
             kind = elem.get("type")
-            if kind != "path":
-                raise ValueError(f"integrator '{kind}' is not supported (path)")
+            if kind not in ("path", "directlighting"):
+                raise ValueError(f"integrator '{kind}' is not supported (path, directlighting)")
             p.add("integrator", kind)
         else:
             kind = elem.get("type")

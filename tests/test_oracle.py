@@ -380,6 +380,34 @@ def test_ref_bxdf(oracle, ref, mat):
                 assert np.allclose(f2, f3, rtol=1e-10, atol=1e-14) and np.allclose(p2, p3, rtol=1e-10)
 
 
+@pytest.mark.parametrize("scene,depth", [("cornell", 8), ("mixed", 16), ("mixed", 3), ("mixed", 1)])
+def test_ref_direct_radiance(oracle, ref, scene, depth):
+    """DirectLightingIntegrator::Li path by path: same MT19937 seed, same radiance
+    (mirrors recurse with depth + 2 per bounce; glass and lights behind mirrors stay dark)."""
+    if scene == "cornell":
+        w = configs.cornell_box()
+        s, cam = w.scene, w.camera
+    else:
+        s, cam = small_mixed_scene()
+    osc, rsc = oracle.scene(s), ref.scene(s)
+    rng = np.random.default_rng(12)
+    n = 1500
+    orgs, dirs = [], []
+    for _ in range(n):
+        x, y = int(rng.integers(0, cam.width)), int(rng.integers(0, cam.height))
+        o, d = oracle.camera_ray(cam, x, y, rng.random(4))
+        orgs.append(o)
+        dirs.append(d)
+    orgs, dirs = np.array(orgs), np.array(dirs)
+    seeds = rng.integers(1, 2 ** 31, n).astype(np.uint32)
+    Lr = rsc.li(orgs, dirs, seeds, depth, direct=True)
+    Lo = np.array([osc.li_mt_direct(orgs[i], dirs[i], int(seeds[i]), depth) for i in range(n)])
+    err = np.abs(Lr - Lo).max(axis=1) / np.maximum(1.0, np.abs(Lr).max(axis=1))
+    assert np.mean(err < 1e-9) > 0.995
+    assert abs(Lr.mean() - Lo.mean()) < 1e-3 * max(1.0, Lr.mean())
+    assert Lr.max() > 0
+
+
 @pytest.mark.parametrize("scene", ["cornell", "mixed"])
 def test_ref_path_radiance(oracle, ref, scene):
     """PathIntegrator::Li path by path: same MT19937 seed, same radiance."""

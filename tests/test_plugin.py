@@ -18,10 +18,11 @@ def test_plugin_exports_spica_entry_points():
     from spica_amd import _abi
 
     _abi.load()   # binds libspk.so to torch's HIP runtime before the plugin pulls it in
-    lib = ctypes.CDLL(PLUGIN)
-    lib.getDescription.restype = ctypes.c_char_p
-    assert b"Path tracing" in lib.getDescription()
-    assert hasattr(lib, "createInstance")
+    for path, text in ((PLUGIN, b"Path tracing"), (build.PLUGIN_DIRECT, b"direct lighting")):
+        lib = ctypes.CDLL(path)
+        lib.getDescription.restype = ctypes.c_char_p
+        assert text in lib.getDescription()
+        assert hasattr(lib, "createInstance")
 
 
 def test_plugin_reports_errors_without_aborting(tmp_path):
@@ -111,3 +112,24 @@ def test_plugin_saves_like_film_save(gpu, tmp_path):
     r = subprocess.run([DEMO, str(sf), str(out), PLUGIN, "out=" + str(tmp_path / "bad%s")], capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 4 and "conversion" in r.stderr
+
+
+@pytest.mark.gpu
+def test_directlighting_plugin_equals_c_abi_render(gpu, tmp_path):
+    """spica's "directlighting" plugin (libspica_directlighting.so) renders the film the C ABI
+    gives with integrator = directlighting, bit for bit."""
+    w = configs.cornell_box(width=64, height=48)
+    sf = tmp_path / "s.bin"
+    w.scene.save_bin(sf, w.camera, 16, 8, 0, 0)
+    out = tmp_path / "img.f64"
+    r = subprocess.run([DEMO, str(sf), str(out), build.PLUGIN_DIRECT], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "direct lighting" in r.stdout
+    img = np.fromfile(out).reshape(48, 64, 3)
+    gs = gpu.GpuScene(w.scene)
+    gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=16, integrator=1))
+    gr.run(16)
+    s, q, c = gr.film()
+    gr.close()
+    gs.close()
+    assert np.array_equal(img, s / (c[..., None] + 1e-12))

@@ -65,6 +65,14 @@ def test_cornell_scene_file_loads_the_programmatic_scene(tmp_path):
     assert (xs.spp, xs.max_depth, xs.sampler, xs.integrator) == (16, 8, _abi.SPK_SAMPLER_INDEPENDENT, "path")
 
 
+def test_directlighting_scene_file(tmp_path):
+    p, w = _cornell_xml(tmp_path)
+    p.write_text(p.read_text().replace('<integrator type="path">', '<integrator type="directlighting">'))
+    xs = sceneio.load_xml(p)
+    assert xs.integrator == "directlighting"
+    assert xs.workload().render_desc().integrator == _abi.SPK_INTEGRATOR_DIRECT
+
+
 def test_transform_composition_and_rotate_in_radians(tmp_path):
     p = tmp_path / "t.xml"
     p.write_text('<scene><transform name="toWorld"><translate x="1" y="2" z="3"/><scale x="2" y="2" z="2"/>'
