@@ -18,28 +18,44 @@ extern "C" {
 #endif
 
 /* Material kinds: the BSDF plugins on the path.
- *   DIFFUSE        bsdfs/diffuse.cc:23-32        LambertianReflection(clamp(reflectance))
- *   CONDUCTOR      bsdfs/conductor.cc:28-38      SpecularReflection + FresnelConductor(1, eta, k)
- *   DIELECTRIC     bsdfs/dielectric.cc:30-42     FresnelSpecular(re, tr, 1, gray(intIOR))
- *   ROUGHCONDUCTOR bsdfs/roughconductor.cc:38-75 MicrofacetReflection + GGX(alpha) + FresnelConductor
+ *   DIFFUSE         bsdfs/diffuse.cc:23-32          LambertianReflection(clamp(reflectance))
+ *   CONDUCTOR       bsdfs/conductor.cc:28-38        SpecularReflection + FresnelConductor(1, eta, k)
+ *   DIELECTRIC      bsdfs/dielectric.cc:30-42       FresnelSpecular(re, tr, 1, gray(intIOR))
+ *   ROUGHCONDUCTOR  bsdfs/roughconductor.cc:38-75   MicrofacetReflection + D(alpha) + FresnelConductor
+ *   PLASTIC         bsdfs/plastic.cc:20-121         PlasticBRDF(Kd, Ks, 1, gray(intIOR))
+ *   ROUGHDIELECTRIC bsdfs/roughdielectric.cc:42-82  MicrofacetTransmission(re, tr, D(alpha), 1, gray(intIOR));
+ *                                                   FresnelSpecular when alpha == 0
+ *   ROUGHPLASTIC    bsdfs/roughplastic.cc:20-160    RoughPlasticBRDF(Kd, Ks, D(alpha), 1, gray(intIOR))
+ * D is the microfacet distribution ("distribution": GGX or Beckmann, spica's default).
  * Spica requires every shape to carry a BSDF (core/material.cc:21-26 dereferences it
  * unconditionally), so there is no "no material" kind. */
 enum spk_material_type {
     SPK_MAT_DIFFUSE = 1,
     SPK_MAT_CONDUCTOR = 2,
     SPK_MAT_DIELECTRIC = 3,
-    SPK_MAT_ROUGHCONDUCTOR = 4
+    SPK_MAT_ROUGHCONDUCTOR = 4,
+    SPK_MAT_PLASTIC = 5,
+    SPK_MAT_ROUGHDIELECTRIC = 6,
+    SPK_MAT_ROUGHPLASTIC = 7
+};
+
+/* "distribution" of the rough materials (core/microfacet.cc:196-356). */
+enum spk_microfacet {
+    SPK_MICROFACET_GGX = 0,
+    SPK_MICROFACET_BECKMANN = 1
 };
 
 typedef struct spk_material {
     int32_t type;             /* enum spk_material_type */
-    int32_t reserved;
-    double reflectance[3];    /* diffuse "reflectance"; dielectric "specularReflectance" */
-    double transmittance[3];  /* dielectric "specularTransmittance" */
+    int32_t distribution;     /* enum spk_microfacet (rough materials) */
+    double reflectance[3];    /* diffuse "reflectance"; (rough)dielectric "specularReflectance";
+                               * (rough)plastic "diffuseReflectance" */
+    double transmittance[3];  /* (rough)dielectric "specularTransmittance" */
     double eta[3];            /* (rough)conductor "eta" */
     double k[3];              /* (rough)conductor "k" */
-    double int_ior;           /* dielectric "intIOR" (spica default 1.333) */
-    double alpha;             /* roughconductor "alpha" (spica default 0.1); 0 -> specular */
+    double int_ior;           /* (rough)dielectric / (rough)plastic "intIOR" */
+    double alpha;             /* rough materials' "alpha" (spica default 0.1); 0 -> specular */
+    double specular[3];       /* (rough)plastic "specularReflectance" */
 } spk_material;
 
 /* Scene: a triangle soup in world space (spica turns every mesh into one Triangle

@@ -29,7 +29,7 @@ REF_UNITS = [
     "core/material", "core/medium", "core/phase", "core/bssrdf", "core/mipmap", "core/interpolation", "core/float",
     "core/exception", "core/uv", "core/constant", "core/tmo", "core/birateral", "core/quaternion",
     "accelerators/bvh", "integrators/path/path", "integrators/directlighting/directlighting", "samplers/independent", "bsdfs/diffuse", "bsdfs/conductor",
-    "bsdfs/dielectric", "bsdfs/roughconductor", "lights/area", "cameras/perspective", "filters/box", "films/hdrfilm",
+    "bsdfs/dielectric", "bsdfs/roughconductor", "bsdfs/plastic", "bsdfs/roughdielectric", "bsdfs/roughplastic", "lights/area", "cameras/perspective", "filters/box", "films/hdrfilm",
 ]
 
 
@@ -72,7 +72,7 @@ def build_ref(jobs=8):
         raise RuntimeError(f"g++ failed on ref_harness.cpp:\n{r.stderr[-2000:]}")
     tmp = REF_LIB + ".tmp"
     # Every spica plugin defines createInstance/getDescription; only one copy is kept.
-    cmd = ["g++", "-shared", "-Wl,--allow-multiple-definition", "-o", tmp] + objs + [hobj, "-lpthread"]
+    cmd = ["g++", "-shared", "-pthread", "-Wl,--allow-multiple-definition", "-o", tmp] + objs + [hobj, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link of libspica_ref.so failed:\n{r.stderr[-2000:]}")

@@ -57,6 +57,7 @@ class Oracle:
         L.orc_env_sample.argtypes = [_vp, ctypes.c_int64, _d, _d, _d, _d]
         L.orc_li_mt.argtypes = [_vp, _d, _d, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _d]
         L.orc_li_mt_direct.argtypes = [_vp, _d, _d, ctypes.c_uint32, ctypes.c_int, _d]
+        L.orc_li_mt_batch.argtypes = [_vp, ctypes.c_int64, _d, _d, _u32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
         L.orc_render.argtypes = [_vp, _vp, _vp, _i32, ctypes.c_int64, ctypes.c_int, _d, _d, _i64, _i64]
         self.L = L
 
@@ -117,9 +118,11 @@ class Oracle:
         return f, pdf.value
 
     def bsdf_sample(self, mat, wo, u2, mask=31):
+        """u2: the two sample uniforms, optionally a third for the BxDF's hidden draw (0.5)."""
         from spica_amd.scene import _material_struct
 
         m = _material_struct(mat)
+        u2 = list(u2) + [0.5] * (3 - len(u2))
         wo, u2 = (np.ascontiguousarray(x, dtype=np.float64) for x in (wo, u2))
         wi = np.zeros(3)
         f = np.zeros(3)
@@ -197,6 +200,17 @@ class OracleScene:
         o, d = (np.ascontiguousarray(x, dtype=np.float64) for x in (org, dirv))
         L = np.zeros(3)
         self.orc.L.orc_li_mt(self.h, _dp(o), _dp(d), seed, max_depth, rr_depth, _dp(L))
+        return L
+
+    def li_mt_batch(self, org, dirv, seeds, max_depth, rr_depth=3, direct=False):
+        """Paths from given rays with spica's MT19937 streams, one seed per path; the BxDFs'
+        hidden streams run on across the batch (spica's thread_local Random on one thread)."""
+        org = np.ascontiguousarray(org, dtype=np.float64)
+        dirv = np.ascontiguousarray(dirv, dtype=np.float64)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+        L = np.zeros((len(org), 3))
+        self.orc.L.orc_li_mt_batch(self.h, len(org), _dp(org), _dp(dirv), seeds.ctypes.data_as(_u32), max_depth, rr_depth,
+                                   1 if direct else 0, _dp(L))
         return L
 
     def li_mt_direct(self, org, dirv, seed, max_depth):

@@ -8,6 +8,7 @@
 // and its Film/Image (which need stb, absent here) are never touched.
 #include <cstdint>
 #include <map>
+#include <thread>
 #include <memory>
 #include <vector>
 
@@ -57,6 +58,21 @@
 #include "integrators/path/path.h"
 #undef createInstance
 #undef getDescription
+#define createInstance createInstance_plastic
+#define getDescription getDescription_plastic
+#include "bsdfs/plastic.h"
+#undef createInstance
+#undef getDescription
+#define createInstance createInstance_roughdielectric
+#define getDescription getDescription_roughdielectric
+#include "bsdfs/roughdielectric.h"
+#undef createInstance
+#undef getDescription
+#define createInstance createInstance_roughplastic
+#define getDescription getDescription_roughplastic
+#include "bsdfs/roughplastic.h"
+#undef createInstance
+#undef getDescription
 #define createInstance createInstance_directlighting
 #define getDescription getDescription_directlighting
 #include "integrators/directlighting/directlighting.h"
@@ -93,6 +109,8 @@ std::shared_ptr<Texture<Spectrum>> tex1(double v) {
     return std::make_shared<ConstantTexture<Spectrum>>(Spectrum(v, v, v));
 }
 
+const char* dist_name(const spk_material& m) { return m.distribution == SPK_MICROFACET_BECKMANN ? "beckmann" : "ggx"; }
+
 std::shared_ptr<SurfaceMaterial> make_surface(const spk_material& m) {
     switch (m.type) {
     case SPK_MAT_DIFFUSE:
@@ -101,8 +119,18 @@ std::shared_ptr<SurfaceMaterial> make_surface(const spk_material& m) {
         return std::make_shared<Conductor>(tex(m.eta), tex(m.k));
     case SPK_MAT_DIELECTRIC:
         return std::make_shared<Dielectric>(tex(m.reflectance), tex(m.transmittance), tex1(m.int_ior));
+    case SPK_MAT_PLASTIC:
+        return std::make_shared<Plastic>(tex(m.reflectance), tex(m.specular), tex1(m.int_ior));
+    case SPK_MAT_ROUGHDIELECTRIC:
+        return std::make_shared<RoughDielectric>(tex(m.reflectance), tex(m.transmittance), tex1(m.alpha), tex1(m.alpha),
+                                                 tex1(m.int_ior), dist_name(m));
+    case SPK_MAT_ROUGHPLASTIC:
+        // the definition (roughplastic.cc:101-108) takes the index before the roughness,
+        // as RoughPlastic(RenderParams&) passes them; the header names them the other way
+        return std::make_shared<RoughPlastic>(tex(m.reflectance), tex(m.specular), tex1(m.int_ior), tex1(m.alpha),
+                                              dist_name(m));
     default:
-        return std::make_shared<RoughConductor>(tex(m.eta), tex(m.k), tex1(m.alpha), tex1(m.alpha), "ggx");
+        return std::make_shared<RoughConductor>(tex(m.eta), tex(m.k), tex1(m.alpha), tex1(m.alpha), dist_name(m));
     }
 }
 
@@ -129,6 +157,8 @@ void li_loop(RefScene* rs, int64_t n, const double* org, const double* dir, cons
                     double* L_out) {
     RenderParams& params = RenderParams::getInstance();
     params.add("maxDepth", max_depth);
+    // on a fresh thread, so the BxDFs' thread_local Random streams start default-seeded
+    std::thread worker([&]() {
     auto dummy = std::make_shared<Independent>(0u);
     H h(dummy);
     MemoryArena arena;
@@ -141,6 +171,8 @@ void li_loop(RefScene* rs, int64_t n, const double* org, const double* dir, cons
         L_out[3 * i + 2] = L.blue();
         arena.reset();
     }
+    });
+    worker.join();
 }
 
 }  // namespace
@@ -222,10 +254,22 @@ static BxDF* make_bxdf(const spk_material& m, MemoryArena& arena) {
         return arena.allocate<FresnelSpecular>(Spectrum(m.reflectance[0], m.reflectance[1], m.reflectance[2]),
                                                Spectrum(m.transmittance[0], m.transmittance[1], m.transmittance[2]), 1.0,
                                                Spectrum(m.int_ior).gray());
+    case SPK_MAT_ROUGHDIELECTRIC: {
+        MicrofacetDistribution* dist =
+            m.distribution == SPK_MICROFACET_BECKMANN
+                ? (MicrofacetDistribution*)arena.allocate<BeckmannDistribution>(m.alpha, m.alpha)
+                : (MicrofacetDistribution*)arena.allocate<TrowbridgeReitzDistribution>(m.alpha, m.alpha);
+        return arena.allocate<MicrofacetTransmission>(Spectrum(m.reflectance[0], m.reflectance[1], m.reflectance[2]),
+                                                      Spectrum(m.transmittance[0], m.transmittance[1], m.transmittance[2]),
+                                                      dist, 1.0, Spectrum(m.int_ior).gray());
+    }
     default: {
         Fresnel* fr = arena.allocate<FresnelConductor>(Spectrum(1.0), Spectrum(m.eta[0], m.eta[1], m.eta[2]),
                                                        Spectrum(m.k[0], m.k[1], m.k[2]));
-        MicrofacetDistribution* dist = arena.allocate<TrowbridgeReitzDistribution>(m.alpha, m.alpha);
+        MicrofacetDistribution* dist =
+            m.distribution == SPK_MICROFACET_BECKMANN
+                ? (MicrofacetDistribution*)arena.allocate<BeckmannDistribution>(m.alpha, m.alpha)
+                : (MicrofacetDistribution*)arena.allocate<TrowbridgeReitzDistribution>(m.alpha, m.alpha);
         return arena.allocate<MicrofacetReflection>(Spectrum(1.0), dist, fr);
     }
     }

@@ -221,18 +221,40 @@ static void make_isect(const Tri* t, const Ray* r, double th, double u, double v
 /* Materials -> single-lobe BSDF (bsdfs/, core/bsdf.cc, core/bxdf.cc)          */
 /* ------------------------------------------------------------------------- */
 enum { BX_REFLECTION = 1, BX_TRANSMISSION = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 16, BX_ALL = 31 };
-enum { LOBE_NONE = 0, LOBE_LAMBERT, LOBE_SPECREFL, LOBE_FRESNELSPEC, LOBE_MICROREFL };
+enum {
+    LOBE_NONE = 0, LOBE_LAMBERT, LOBE_SPECREFL, LOBE_FRESNELSPEC, LOBE_MICROREFL,
+    LOBE_PLASTIC,      /* PlasticBRDF (plastic.cc:20-87) */
+    LOBE_MICROTRANS,   /* MicrofacetTransmission (bxdf.cc:301-423) */
+    LOBE_ROUGHPLASTIC  /* RoughPlasticBRDF (roughplastic.cc:21-101) */
+};
 
 typedef struct {
     int lobe;       /* LOBE_* ; LOBE_NONE = BSDF with zero BxDFs */
     int type;       /* BxDF type bits */
-    V3 R;           /* Lambert reflectance / specular ref_ / microfacet ref_ (1) */
-    V3 T;           /* FresnelSpecular tr_ */
+    V3 R;           /* Lambert reflectance / specular ref_ / microfacet ref_ (1) / re_ / Kd */
+    V3 T;           /* FresnelSpecular and MicrofacetTransmission tr_ */
+    V3 Ks;          /* (rough) plastic Ks */
+    double specW;   /* (rough) plastic specularSamplingWeight_ */
     double etaA, etaB;
     V3 cEta, cK;    /* FresnelConductor(etaI = 1, etaT = eta, k) */
-    double ax, ay;  /* GGX alphas */
+    double ax, ay;  /* microfacet alphas */
+    int dist;       /* SPK_MICROFACET_GGX / SPK_MICROFACET_BECKMANN */
     V3 n, t, b;     /* BSDF frame (bsdf.cc:14-21) */
 } BSDF;
+
+/* The hidden uniform draws of MicrofacetTransmission::sample (bxdf.cc:364),
+ * PlasticBRDF::sample (plastic.cc:54) and RoughPlasticBRDF::sample (roughplastic.cc:58):
+ * spica takes them from a thread_local Random per source file, default-seeded (0); with
+ * the MT19937 sampler they come from three such streams, otherwise from one sampler
+ * dimension the caller sets in `u`. */
+enum { HID_BXDF = 0, HID_PLASTIC = 1, HID_ROUGHPLASTIC = 2 };
+typedef struct MT MT;
+typedef struct {
+    int use_mt;
+    MT* mt[3];
+    double u;
+} Hidden;
+static double hidden_draw(Hidden* h, int stream);
 
 /* bsdfs/diffuse.cc:23-32, conductor.cc:28-38, dielectric.cc:30-42, roughconductor.cc:38-75 */
 static void setup_bsdf(const spk_material* m, const Isect* is, BSDF* b) {
@@ -277,6 +299,7 @@ static void setup_bsdf(const spk_material* m, const Isect* is, BSDF* b) {
         b->cEta = v3(m->eta[0], m->eta[1], m->eta[2]);
         b->cK = v3(m->k[0], m->k[1], m->k[2]);
         b->R = v3(1, 1, 1);
+        b->dist = m->distribution;
         if (a == 0.0) {
             b->lobe = LOBE_SPECREFL;
             b->type = BX_REFLECTION | BX_SPECULAR;
@@ -286,6 +309,64 @@ static void setup_bsdf(const spk_material* m, const Isect* is, BSDF* b) {
             b->ax = a;
             b->ay = a;
         }
+        break;
+    }
+    case SPK_MAT_PLASTIC: {  /* plastic.cc:111-121: no clamping */
+        V3 ior = v3(m->int_ior, m->int_ior, m->int_ior);
+        b->lobe = LOBE_PLASTIC;
+        b->type = BX_REFLECTION | BX_SPECULAR | BX_DIFFUSE;
+        b->R = v3(m->reflectance[0], m->reflectance[1], m->reflectance[2]);
+        b->Ks = v3(m->specular[0], m->specular[1], m->specular[2]);
+        b->etaA = 1.0;
+        b->etaB = gray(ior);
+        b->specW = gray(b->Ks) / (gray(b->R) + gray(b->Ks));
+        break;
+    }
+    case SPK_MAT_ROUGHDIELECTRIC: {  /* roughdielectric.cc:42-82 */
+        V3 ior = v3(m->int_ior, m->int_ior, m->int_ior);
+        V3 al = v3(m->alpha, m->alpha, m->alpha);
+        double eta = gray(ior), a = gray(al);
+        V3 re = spec_clamp(v3(m->reflectance[0], m->reflectance[1], m->reflectance[2]));
+        V3 tr = spec_clamp(v3(m->transmittance[0], m->transmittance[1], m->transmittance[2]));
+        if (vis_black(re) && vis_black(tr)) break;
+        b->R = re;
+        b->T = tr;
+        b->etaA = 1.0;
+        b->etaB = eta;
+        if (a == 0.0) {
+            b->lobe = LOBE_FRESNELSPEC;
+            b->type = BX_REFLECTION | BX_TRANSMISSION | BX_SPECULAR;
+        } else {
+            b->lobe = LOBE_MICROTRANS;
+            b->type = BX_REFLECTION | BX_TRANSMISSION | BX_GLOSSY;
+            b->ax = a;
+            b->ay = a;
+            b->dist = m->distribution;
+        }
+        break;
+    }
+    case SPK_MAT_ROUGHPLASTIC: {  /* roughplastic.cc:128-158 */
+        V3 ior = v3(m->int_ior, m->int_ior, m->int_ior);
+        V3 al = v3(m->alpha, m->alpha, m->alpha);
+        V3 kd = v3(m->reflectance[0], m->reflectance[1], m->reflectance[2]);
+        V3 ks = v3(m->specular[0], m->specular[1], m->specular[2]);
+        double rough = fmax(0.001, gray(al));
+        if (vis_black(ks)) {
+            b->lobe = LOBE_LAMBERT;
+            b->type = BX_REFLECTION | BX_DIFFUSE;
+            b->R = kd;
+            break;
+        }
+        b->lobe = LOBE_ROUGHPLASTIC;
+        b->type = BX_REFLECTION | BX_DIFFUSE | BX_GLOSSY;
+        b->R = kd;
+        b->Ks = ks;
+        b->etaA = 1.0;
+        b->etaB = gray(ior);
+        b->specW = gray(ks) / (gray(kd) + gray(ks));
+        b->ax = rough;
+        b->ay = rough;
+        b->dist = m->distribution;
         break;
     }
     default:
@@ -453,6 +534,143 @@ static V3 ggx_sample_wh(const BSDF* b, V3 wo, double u0, double u1) {
     return wh;
 }
 
+/* math.h:14-42: erfinv, the single-precision coefficients of Giles, "Approximating the
+ * erfinv function" (GPU Computing Gems, 2011), evaluated in double as spica does (the
+ * float literals are rounded to float first). */
+static double math_erfinv(double x) {
+    double w, p;
+    x = clampd(x, -0.99999, 0.99999);
+    w = -log((1 - x) * (1 + x));
+    if (w < 5) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = 3.43273939e-07f + p * w;
+        p = -3.5233877e-06f + p * w;
+        p = -4.39150654e-06f + p * w;
+        p = 0.00021858087f + p * w;
+        p = -0.00125372503f + p * w;
+        p = -0.00417768164f + p * w;
+        p = 0.246640727f + p * w;
+        p = 1.50140941f + p * w;
+    } else {
+        w = sqrt(w) - 3;
+        p = -0.000200214257f;
+        p = 0.000100950558f + p * w;
+        p = 0.00134934322f + p * w;
+        p = -0.00367342844f + p * w;
+        p = 0.00573950773f + p * w;
+        p = -0.0076224613f + p * w;
+        p = 0.00943887047f + p * w;
+        p = 1.00167406f + p * w;
+        p = 2.83297682f + p * w;
+    }
+    return p * x;
+}
+/* math.h:44-65: erf by Abramowitz & Stegun 7.1.26 (float-rounded coefficients) */
+static double math_erf(double x) {
+    const double a1 = 0.254829592f, a2 = -0.284496736f, a3 = 1.421413741f;
+    const double a4 = -1.453152027f, a5 = 1.061405429f, pp = 0.3275911f;
+    int sign = x < 0 ? -1 : 1;
+    x = fabs(x);
+    double t = 1 / (1 + pp * x);
+    double y = 1 - (((((a5 * t + a4) * t) + a3) * t + a2) * t + a1) * t * exp(-x * x);
+    return sign * y;
+}
+/* microfacet.cc:277-288 */
+static double bk_D(const BSDF* b, V3 wh) {
+    double t2 = tan2_theta(wh);
+    if (isinf(t2)) return 0.0;
+    double c4 = cos2_theta(wh) * cos2_theta(wh);
+    double ab2 = cos2_phi(wh) / (b->ax * b->ax) + sin2_phi(wh) / (b->ay * b->ay);
+    return exp(-t2 * ab2) / (PI * b->ax * b->ay * c4);
+}
+/* microfacet.cc:342-355: Walter et al. 2007's rational fit of the Smith lambda */
+static double bk_lambda(const BSDF* b, V3 w) {
+    double at = fabs(tan_theta(w));
+    if (isinf(at)) return 0.0;
+    double alpha = sqrt(cos2_phi(w) * b->ax * b->ax + sin2_phi(w) * b->ay * b->ay);
+    double a = 1.0 / (alpha * at);
+    if (a >= 1.6) return 0.0;
+    return (1.0 - 1.259 * a + 0.396 * a * a) / (3.535 * a + 2.181 * a * a);
+}
+/* microfacet.cc:80-131: visible slopes by Jakob 2014's bisection-Newton on C22 */
+static void bk_sample11(double cosI, double U1, double U2, double* sx, double* sy) {
+    if (cosI > 0.9999) {
+        double r = sqrt(-log(1.0 - U1));
+        double sp = sin(2.0 * PI * U2), cp = cos(2.0 * PI * U2);
+        *sx = r * cp;
+        *sy = r * sp;
+        return;
+    }
+    double sinI = sqrt(fmax(0.0, 1.0 - cosI * cosI));
+    double tanI = sinI / cosI;
+    double cotI = 1.0 / tanI;
+    double a = -1.0;
+    double c = math_erf(cotI);
+    double sx0 = fmax(U1, 1.0e-6);
+    double thetaI = acos(cosI);
+    double fit = 1.0 + thetaI * (-0.876 + thetaI * (0.4265 - 0.0594 * thetaI));
+    double bb = c - (1.0 + c) * pow(1.0 - sx0, fit);
+    const double SQRT_PI_INV = 1.0 / sqrt(PI);
+    double norm = 1.0 / (1.0 + c + SQRT_PI_INV * tanI * exp(-cotI * cotI));
+    for (int it = 0; it < 16; it++) {
+        if (bb < a || c < bb) bb = 0.5 * (a + c);
+        double xm = math_erfinv(bb);
+        double value = norm * (1.0 + bb + SQRT_PI_INV * tanI * exp(-xm * xm)) - sx0;
+        double deriv = norm * (1.0 - xm * tanI);
+        if (fabs(value) < 1.0e-6) break;
+        if (value > 0.0) c = bb;
+        else a = bb;
+        bb -= value / deriv;
+    }
+    *sx = math_erfinv(bb);
+    *sy = math_erfinv(2.0 * fmax(U2, 1.0e-6) - 1.0);
+}
+/* microfacet.cc:133-157 */
+static V3 bk_sample_tr(V3 wi, double ax, double ay, double u0, double u1) {
+    V3 ws = vnormalize(v3(ax * wi.x, ay * wi.y, wi.z));
+    double sx, sy;
+    bk_sample11(cos_theta(ws), u0, u1, &sx, &sy);
+    double cp = cos_phi(ws), sp = sin_phi(ws);
+    double tmp = cp * sx - sp * sy;
+    sy = sp * sx + cp * sy;
+    sx = tmp;
+    sx *= ax;
+    sy *= ay;
+    return vnormalize(v3(-sx, -sy, 1.0));
+}
+
+/* MicrofacetDistribution with the material's distribution (microfacet.cc:166-193) */
+static double mf_D(const BSDF* b, V3 wh) { return b->dist == SPK_MICROFACET_BECKMANN ? bk_D(b, wh) : ggx_D(b, wh); }
+static double mf_lambda(const BSDF* b, V3 w) {
+    return b->dist == SPK_MICROFACET_BECKMANN ? bk_lambda(b, w) : ggx_lambda(b, w);
+}
+static double mf_G1(const BSDF* b, V3 w, V3 wh) {
+    if (w.z * vdot(w, wh) <= 0.0) return 0.0;
+    return 1.0 / (1.0 + mf_lambda(b, w));
+}
+static double mf_G(const BSDF* b, V3 wo, V3 wi, V3 wh) {
+    if (wi.z * vdot(wi, wh) <= 0.0) return 0.0;
+    if (wo.z * vdot(wo, wh) <= 0.0) return 0.0;
+    return 1.0 / (1.0 + mf_lambda(b, wo) + mf_lambda(b, wi));
+}
+static double mf_pdf(const BSDF* b, V3 wo, V3 wh) {
+    return mf_D(b, wh) * mf_G1(b, wo, wh) * vabsdot(wo, wh) / fabs(cos_theta(wo));
+}
+static V3 mf_sample_wh(const BSDF* b, V3 wo, double u0, double u1) {
+    if (b->dist != SPK_MICROFACET_BECKMANN) return ggx_sample_wh(b, wo, u0, u1);
+    int flip = wo.z < 0.0;
+    V3 wh = bk_sample_tr(flip ? vneg(wo) : wo, b->ax, b->ay, u0, u1);
+    if (flip) wh = vneg(wh);
+    return wh;
+}
+
+/* fresnel.cc:87-95 */
+static double fr_diffuse_reflectance(double eta) {
+    if (eta >= 1.0) return -1.4399 / (eta * eta) + 0.7099 / eta + 0.6681 + 0.0636 * eta;
+    return -0.4399f + 0.7099 / eta - 0.3319 / (eta * eta) + 0.0636 / (eta * eta * eta);
+}
+
 /* bxdf.cc:264-278 */
 static V3 micro_f(const BSDF* b, V3 wo, V3 wi) {
     double cO = fabs(cos_theta(wo)), cI = fabs(cos_theta(wi));
@@ -461,8 +679,49 @@ static V3 micro_f(const BSDF* b, V3 wo, V3 wi) {
     if (wh.x == 0.0 && wh.y == 0.0 && wh.z == 0.0) return v3(0, 0, 0);
     wh = vnormalized(wh);
     V3 F = fr_conductor(fabs(vdot(wi, wh)), v3(1, 1, 1), b->cEta, b->cK);
-    double s = ggx_D(b, wh) * ggx_G(b, wo, wi, wh) / (4.0 * cI * cO);
+    double s = mf_D(b, wh) * mf_G(b, wo, wi, wh) / (4.0 * cI * cO);
     return vmul(vcmul(b->R, F), s);
+}
+
+/* bxdf.cc:333-354: MicrofacetTransmission::f(wo, wi, wh) */
+static V3 mtrans_f_wh(const BSDF* b, V3 wo, V3 wi, V3 wh) {
+    double cO = cos_theta(wo), cI = cos_theta(wi);
+    if (cO == 0.0 || cI == 0.0) return v3(0, 0, 0);
+    if (wo.z * vdot(wo, wh) <= 0.0) return v3(0, 0, 0);
+    if (wi.z * vdot(wi, wh) <= 0.0) return v3(0, 0, 0);
+    double F = fr_dielectric(vdot(wo, wh), b->etaA, b->etaB);
+    if (same_hemi(wo, wi)) return vmul(b->R, F * mf_D(b, wh) * mf_G(b, wo, wi, wh) / (4.0 * cO * cI));
+    double eta = vdot(wo, wh) > 0.0 ? (b->etaB / b->etaA) : (b->etaA / b->etaB);
+    double factor = 1.0 / eta;
+    double sd = vdot(wo, wh) + eta * vdot(wi, wh);
+    return vmul(b->T, (1.0 - F) * fabs(mf_D(b, wh) * mf_G(b, wo, wi, wh) * eta * eta * vabsdot(wi, wh) *
+                                       vabsdot(wo, wh) * factor * factor / (cI * cO * sd * sd)));
+}
+/* bxdf.cc:408-423 */
+static double mtrans_pdf_wh(const BSDF* b, V3 wo, V3 wi, V3 wh) {
+    if (wo.z * vdot(wo, wh) <= 0.0) return 0.0;
+    if (wi.z * vdot(wi, wh) <= 0.0) return 0.0;
+    double F = fr_dielectric(vdot(wo, wh), b->etaA, b->etaB);
+    if (same_hemi(wo, wi)) return F * mf_pdf(b, wo, wh) / (4.0 * vabsdot(wo, wh));
+    double eta = vdot(wo, wh) > 0.0 ? (b->etaB / b->etaA) : (b->etaA / b->etaB);
+    double sd = vdot(wo, wh) + eta * vdot(wi, wh);
+    double dwh = fabs((eta * eta * vdot(wi, wh)) / (sd * sd));
+    return (1.0 - F) * mf_pdf(b, wo, wh) * dwh;
+}
+/* bxdf.cc:315-331, 394-406: the half vector of a (wo, wi) pair, on the +z side */
+static V3 mtrans_wh(const BSDF* b, V3 wo, V3 wi) {
+    V3 wh;
+    if (same_hemi(wo, wi)) {
+        wh = vnormalize(vadd(wo, wi));
+    } else {
+        double eta = cos_theta(wo) > 0.0 ? (b->etaB / b->etaA) : (b->etaA / b->etaB);
+        wh = vnormalize(vadd(wo, vmul(wi, eta)));
+    }
+    return wh.z >= 0.0 ? wh : vneg(wh);
+}
+/* plastic.cc:28 / roughplastic.cc:28: probability of the specular branch */
+static double plastic_prob_spec(const BSDF* b, double Fo) {
+    return (Fo * b->specW) / (Fo * b->specW + (1.0 - Fo) * (1.0 - b->specW));
 }
 
 /* BxDF::f for the lobe (bxdf.cc:56-58, 86-91, 171-192, 264-278) */
@@ -490,6 +749,27 @@ static V3 lobe_f(const BSDF* b, V3 wo, V3 wi) {
     }
     case LOBE_MICROREFL:
         return micro_f(b, wo, wi);
+    case LOBE_PLASTIC: {  /* plastic.cc:32-46; the "specular" test compares wi with its own mirror image */
+        double Fo = fr_dielectric(cos_theta(wo), b->etaA, b->etaB);
+        if (vdot(wi, v3(-wi.x, -wi.y, wi.z)) > 1.0 - EPS) return vmul(b->Ks, Fo / fabs(cos_theta(wi)));
+        double eta = b->etaB / b->etaA, invEta = 1.0 / eta;
+        double Fi = fr_dielectric(cos_theta(wi), b->etaA, b->etaB);
+        double Fdr = fr_diffuse_reflectance(eta);
+        return vmul(b->R, (1.0 - Fo) * (1.0 - Fi) * invEta * invEta * INV_PI / (1.0 - Fdr));
+    }
+    case LOBE_MICROTRANS:  /* bxdf.cc:315-331 */
+        if (cos_theta(wo) == 0.0 || cos_theta(wi) == 0.0) return v3(0, 0, 0);
+        return mtrans_f_wh(b, wo, wi, mtrans_wh(b, wo, wi));
+    case LOBE_ROUGHPLASTIC: {  /* roughplastic.cc:35-47 */
+        double cO = cos_theta(wo), cI = cos_theta(wi);
+        if (cO == 0.0 || cI == 0.0) return v3(0, 0, 0);
+        if (!same_hemi(wo, wi)) return v3(0, 0, 0);
+        V3 wh = vnormalize(vadd(wi, wo));
+        double F = fr_dielectric(vdot(wo, wh), b->etaA, b->etaB);
+        V3 spec = vmul(b->Ks, F * mf_D(b, wh) * mf_G(b, wo, wi, wh) / (4.0 * cO * cI));
+        V3 diff = vmul(b->R, (1.0 - F) * INV_PI);
+        return vadd(spec, diff);
+    }
     default:
         return v3(0, 0, 0);
     }
@@ -520,7 +800,23 @@ static double lobe_pdf(const BSDF* b, V3 wo, V3 wi) {
     case LOBE_MICROREFL: {
         if (!same_hemi(wo, wi)) return 0.0;
         V3 wh = vnormalize(vadd(wo, wi));
-        return ggx_pdf(b, wo, wh) / (4.0 * vdot(wo, wh));
+        return mf_pdf(b, wo, wh) / (4.0 * vdot(wo, wh));
+    }
+    case LOBE_PLASTIC: {  /* plastic.cc:71-82 */
+        double Fo = fr_dielectric(cos_theta(wo), b->etaA, b->etaB);
+        double ps = plastic_prob_spec(b, Fo);
+        if (vdot(wi, v3(-wi.x, -wi.y, wi.z)) > 1.0 - EPS) return ps;
+        return (1.0 - ps) * fabs(cos_theta(wi)) * INV_PI;
+    }
+    case LOBE_MICROTRANS:  /* bxdf.cc:394-406 */
+        return mtrans_pdf_wh(b, wo, wi, mtrans_wh(b, wo, wi));
+    case LOBE_ROUGHPLASTIC: {  /* roughplastic.cc:88-97 (F-weighted, not the sampling weights) */
+        if (!same_hemi(wo, wi)) return 0.0;
+        V3 wh = vnormalize(vadd(wi, wo));
+        double F = fr_dielectric(vdot(wo, wh), b->etaA, b->etaB);
+        double sp = F * mf_pdf(b, wo, wh) / (4.0 * vabsdot(wo, wh));
+        double dp = (1.0 - F) * fabs(cos_theta(wi)) * INV_PI;
+        return sp + dp;
     }
     default:
         return 0.0;
@@ -555,7 +851,7 @@ static V3 cosine_hemisphere(double u0, double u1) {
 
 /* BxDF::sample for the lobe (bxdf.cc:34-40, 93-101, 194-225, 280-291).
  * Returns f; *pdf stays 0 on failure (the caller zeroes it, bsdf.cc:244). */
-static V3 lobe_sample(const BSDF* b, V3 wo, V3* wi, double u0, double u1, double* pdf, int* stype) {
+static V3 lobe_sample(const BSDF* b, V3 wo, V3* wi, double u0, double u1, double* pdf, int* stype, Hidden* hu) {
     switch (b->lobe) {
     case LOBE_LAMBERT: {
         *wi = cosine_hemisphere(u0, u1);
@@ -590,11 +886,64 @@ static V3 lobe_sample(const BSDF* b, V3 wo, V3* wi, double u0, double u1, double
     }
     case LOBE_MICROREFL: {
         if (wo.z == 0.0) return v3(0, 0, 0);
-        V3 wh = ggx_sample_wh(b, wo, u0, u1);
+        V3 wh = mf_sample_wh(b, wo, u0, u1);
         *wi = reflect_v(wo, wh);
         if (!same_hemi(wo, *wi)) return v3(0, 0, 0);
-        *pdf = ggx_pdf(b, wo, wh) / (4.0 * vdot(wo, wh));
+        *pdf = mf_pdf(b, wo, wh) / (4.0 * vdot(wo, wh));
         return micro_f(b, wo, *wi);
+    }
+    case LOBE_PLASTIC: {  /* plastic.cc:48-69 */
+        double Fo = fr_dielectric(cos_theta(wo), b->etaA, b->etaB);
+        double ps = plastic_prob_spec(b, Fo);
+        if (hidden_draw(hu, HID_PLASTIC) < ps) {
+            *wi = v3(-wo.x, -wo.y, wo.z);
+            *pdf = ps;
+            return vmul(b->Ks, Fo / fabs(cos_theta(*wi)));
+        }
+        *wi = cosine_hemisphere(u0, u1);
+        double eta = b->etaB / b->etaA, invEta = 1.0 / eta;
+        double Fi = fr_dielectric(cos_theta(*wi), b->etaA, b->etaB);
+        double Fdr = fr_diffuse_reflectance(eta);
+        *pdf = (1.0 - ps) * fabs(cos_theta(*wi)) * INV_PI;
+        return vmul(b->R, (1.0 - Fo) * (1.0 - Fi) * invEta * invEta * INV_PI / (1.0 - Fdr));
+    }
+    case LOBE_MICROTRANS: {  /* bxdf.cc:356-392 */
+        if (wo.z == 0.0) return v3(0, 0, 0);
+        V3 wh = mf_sample_wh(b, wo, u0, u1);
+        double cO = vdot(wo, wh.z > 0.0 ? wh : vneg(wh));
+        double F = fr_dielectric(cO, b->etaA, b->etaB);
+        if (hidden_draw(hu, HID_BXDF) < F) {
+            *wi = reflect_v(wo, wh);
+            if (!same_hemi(wo, *wi)) return v3(0, 0, 0);
+            *stype = BX_REFLECTION | BX_GLOSSY;
+        } else {
+            double eta = cO > 0.0 ? (b->etaA / b->etaB) : (b->etaB / b->etaA);
+            if (!refract_v(wo, wh, eta, wi)) return v3(0, 0, 0);
+            if (same_hemi(wo, *wi)) return v3(0, 0, 0);
+            *stype = BX_TRANSMISSION | BX_GLOSSY;
+        }
+        V3 whp = wh.z >= 0.0 ? wh : vneg(wh);
+        *pdf = mtrans_pdf_wh(b, wo, *wi, whp);
+        return mtrans_f_wh(b, wo, *wi, whp);
+    }
+    case LOBE_ROUGHPLASTIC: {  /* roughplastic.cc:49-80: the diffuse branch reuses the wh sample's uniforms */
+        if (wo.z == 0.0) return v3(0, 0, 0);
+        V3 wh = mf_sample_wh(b, wo, u0, u1);
+        double Fo = fr_dielectric(vdot(wo, wh), b->etaA, b->etaB);
+        double ps = plastic_prob_spec(b, Fo);
+        if (hidden_draw(hu, HID_ROUGHPLASTIC) < ps) {
+            *wi = reflect_v(wo, wh);
+            if (!same_hemi(wo, *wi)) return v3(0, 0, 0);
+            *pdf = ps * mf_pdf(b, wo, wh) / (4.0 * vabsdot(wo, wh));
+            double cO = cos_theta(wo), cI = cos_theta(*wi);
+            return vmul(b->Ks, Fo * mf_D(b, wh) * mf_G(b, wo, *wi, wh) / (4.0 * cO * cI));
+        }
+        *wi = cosine_hemisphere(u0, u1);
+        *pdf = (1.0 - ps) * fabs(cos_theta(*wi)) * INV_PI;
+        double eta = b->etaB / b->etaA, invEta = 1.0 / eta;
+        double Fi = fr_dielectric(cos_theta(*wi), b->etaA, b->etaB);
+        double Fdr = fr_diffuse_reflectance(eta);
+        return vmul(b->R, (1.0 - Fo) * (1.0 - Fi) * invEta * invEta * INV_PI / (1.0 - Fdr));
     }
     default:
         return v3(0, 0, 0);
@@ -628,7 +977,8 @@ static double bsdf_pdf(const BSDF* b, V3 woW, V3 wiW, int mask) {
 }
 /* bsdf.cc:65-122 with a single BxDF.  *pdf is left untouched when wo.z == 0
  * (spica returns before zeroing it); callers initialise it to 0. */
-static V3 bsdf_sample(const BSDF* b, V3 woW, V3* wiW, double u0, double u1, double* pdf, int mask, int* stype) {
+static V3 bsdf_sample(const BSDF* b, V3 woW, V3* wiW, double u0, double u1, double* pdf, int mask, int* stype,
+                      Hidden* hu) {
     int match = bsdf_num_components(b, mask);
     if (match == 0) {
         *pdf = 0.0;
@@ -641,7 +991,7 @@ static V3 bsdf_sample(const BSDF* b, V3 woW, V3* wiW, double u0, double u1, doub
     *pdf = 0.0;
     *stype = b->type;
     V3 wi = v3(0, 0, 0);
-    V3 f = lobe_sample(b, wo, &wi, ur0, u1, pdf, stype);
+    V3 f = lobe_sample(b, wo, &wi, ur0, u1, pdf, stype, hu);
     if (*pdf == 0.0) {
         *stype = 0;
         return v3(0, 0, 0);
@@ -1251,7 +1601,7 @@ static int scene_any(const Scene* s, const Ray* r, Counts* cnt) {
 /* Samplers                                                                   */
 /* ------------------------------------------------------------------------- */
 /* MT19937 (random.cc:104-164) */
-typedef struct { uint32_t mt[624]; int mti; } MT;
+struct MT { uint32_t mt[624]; int mti; };
 static void mt_seed(MT* m, uint32_t s) {
     m->mt[0] = s;
     for (m->mti = 1; m->mti < 624; m->mti++)
@@ -1372,11 +1722,32 @@ typedef struct {
     int kind;
     uint32_t seed, pixel, sample;
     MT* mt;
+    Hidden* hid;   /* the BxDFs' hidden uniforms */
 } Sampler;
 
 static inline double smp(Sampler* s, uint32_t dim) {
     if (s->kind == SPK_SAMPLER_MT19937) return mt_real2(s->mt);
     return u32_to_unit(counter_u32(s->kind, s->seed, s->pixel, s->sample, dim));
+}
+
+static double hidden_draw(Hidden* h, int stream) { return h->use_mt ? mt_real2(h->mt[stream]) : h->u; }
+
+/* With counter-based samplers the hidden uniform of a BSDF sample is one more
+ * dimension: for the path integrator 4 + 8 * 128 + 2 * bounce + (0 NEE, 1 continuation),
+ * past the path's own 8 per bounce and the learned proposal's block; for directlighting
+ * the last of its 10 per level. */
+static inline uint32_t hidden_dim_path(int bounce, int which) { return 4u + 8u * 128u + 2u * (uint32_t)bounce + (uint32_t)which; }
+static inline void hidden_set(Sampler* sp, uint32_t dim) {
+    if (sp->kind != SPK_SAMPLER_MT19937) sp->hid->u = smp(sp, dim);
+}
+/* three default-seeded thread-local streams, as spica's Random() (random.cc:70-74) */
+static void hidden_init_mt(Hidden* h, MT* storage) {
+    h->use_mt = 1;
+    for (int k = 0; k < 3; k++) {
+        mt_seed(&storage[k], 0u);
+        h->mt[k] = &storage[k];
+    }
+    h->u = 0.0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1409,7 +1780,7 @@ static inline double power_heuristic(double f, double g) { return (f * f) / (f *
 
 /* mis.cc:35-136 (surface interaction, no media, non-delta lights) */
 static V3 estimate_direct(const Scene* s, const Isect* is, const BSDF* b, int li, double ul0, double ul1, double us0,
-                          double us1, Counts* cnt) {
+                          double us1, Counts* cnt, Sampler* sp, uint32_t hdim) {
     const int mask = BX_ALL & ~BX_SPECULAR;
     V3 Ld = v3(0, 0, 0);
     V3 wi = v3(0, 0, 0);
@@ -1453,7 +1824,8 @@ static V3 estimate_direct(const Scene* s, const Isect* is, const BSDF* b, int li
         int st = 0;
         double pdf = 0.0;
         V3 wi2 = v3(0, 0, 0);
-        V3 f = bsdf_sample(b, is->wo, &wi2, us0, us1, &pdf, mask, &st);
+        hidden_set(sp, hdim);
+        V3 f = bsdf_sample(b, is->wo, &wi2, us0, us1, &pdf, mask, &st, sp->hid);
         f = vmul(f, vabsdot(wi2, b->n));
         bsdfPdf = pdf;
         int sampledSpecular = (st & BX_SPECULAR) != 0;
@@ -1516,14 +1888,15 @@ static V3 Li_path(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int rrDept
             if (li > s->nlights - 1) li = s->nlights - 1;
             double l0 = smp(sp, base + 1), l1 = smp(sp, base + 2);
             double s0 = smp(sp, base + 3), s1 = smp(sp, base + 4);
-            V3 Ld = estimate_direct(s, &is, &b, li, l0, l1, s0, s1, cp);
+            V3 Ld = estimate_direct(s, &is, &b, li, l0, l1, s0, s1, cp, sp, hidden_dim_path(bounces, 0));
             L = vadd(L, vcmul(beta, vmul(Ld, (double)s->nlights)));
         }
         V3 wo = vneg(ray.d), wi = v3(0, 0, 0);
         double pdf = 0.0;
         int st = 0;
         double c0 = smp(sp, base + 5), c1 = smp(sp, base + 6);
-        V3 f = bsdf_sample(&b, wo, &wi, c0, c1, &pdf, BX_ALL, &st);
+        hidden_set(sp, hidden_dim_path(bounces, 1));
+        V3 f = bsdf_sample(&b, wo, &wi, c0, c1, &pdf, BX_ALL, &st, sp->hid);
         if (vis_black(f) || pdf == 0.0) break;
         beta = vcmul(beta, vmul(f, vabsdot(wi, is.ns) / pdf));
         specular = (st & BX_SPECULAR) != 0;
@@ -1549,7 +1922,8 @@ static V3 Li_path(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int rrDept
  * (MT19937): light pick, light uv, BSDF uv (uniformSampleOneLight, always drawn when the
  * scene has lights), then while depth + 1 < maxDepth the reflection uv (and its whole
  * subtree), then the transmission uv.  Counter-based dimensions: a block of 10 per
- * recursion level (pick 0, light 1-2, BSDF 3-4, reflection 5-6, transmission 7-8); the
+ * recursion level (pick 0, light 1-2, BSDF 3-4, reflection 5-6, transmission 7-8, the
+ * NEE BSDF sample's hidden uniform 9); the
  * supported lobes continue through at most one of the two specular calls (a
  * FresnelSpecular lobe, type R|T|S, is no subset of R|S or T|S, so BSDF::sample finds no
  * component for either, bsdf.cc:28-36,65-73), so one block per level suffices. */
@@ -1576,7 +1950,7 @@ static V3 Li_direct(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int dept
         if (li > s->nlights - 1) li = s->nlights - 1;
         double l0 = smp(sp, base + 1), l1 = smp(sp, base + 2);
         double s0 = smp(sp, base + 3), s1 = smp(sp, base + 4);
-        V3 Ld = estimate_direct(s, &is, &b, li, l0, l1, s0, s1, cp);
+        V3 Ld = estimate_direct(s, &is, &b, li, l0, l1, s0, s1, cp, sp, base + 9);
         L = vadd(L, vmul(Ld, (double)s->nlights));
     }
     if (depth + 1 < maxDepth) {
@@ -1586,7 +1960,7 @@ static V3 Li_direct(const Scene* s, Ray ray, Sampler* sp, int maxDepth, int dept
             V3 wi = v3(0, 0, 0);
             double pdf = 0.0;
             int st = 0;
-            V3 f = bsdf_sample(&b, is.wo, &wi, u0, u1, &pdf, masks[c], &st);
+            V3 f = bsdf_sample(&b, is.wo, &wi, u0, u1, &pdf, masks[c], &st, sp->hid);
             double cosg = vabsdot(wi, is.n);
             if (pdf > 0.0 && !vis_black(f) && cosg != 0.0) {
                 Ray r = make_ray(offset_ray_origin(is.pos, is.n, wi), wi, INFTY);
@@ -1893,14 +2267,16 @@ EXPORT void orc_bsdf_eval(const spk_material* m, const double* wo, const double*
     f_out[2] = f.z;
     *pdf_out = bsdf_pdf(&b, v3(wo[0], wo[1], wo[2]), v3(wi[0], wi[1], wi[2]), mask);
 }
-EXPORT int orc_bsdf_sample(const spk_material* m, const double* wo, const double* u2, int mask, double* wi_out,
+/* u3: the two sample uniforms and the BxDF's hidden uniform */
+EXPORT int orc_bsdf_sample(const spk_material* m, const double* wo, const double* u3, int mask, double* wi_out,
                            double* f_out, double* pdf_out) {
     BSDF b;
     local_bsdf(m, &b);
     V3 wi = v3(0, 0, 0);
     double pdf = 0.0;
     int st = 0;
-    V3 f = bsdf_sample(&b, v3(wo[0], wo[1], wo[2]), &wi, u2[0], u2[1], &pdf, mask, &st);
+    Hidden hu = {0, {NULL, NULL, NULL}, u3[2]};
+    V3 f = bsdf_sample(&b, v3(wo[0], wo[1], wo[2]), &wi, u3[0], u3[1], &pdf, mask, &st, &hu);
     wi_out[0] = wi.x;
     wi_out[1] = wi.y;
     wi_out[2] = wi.z;
@@ -1952,33 +2328,38 @@ EXPORT void orc_env_sample(void* p, int64_t n, const double* u2, double* dir_out
     }
 }
 
-/* One path with spica's MT19937 Independent sampler seeded `seed`, starting from
- * a given ray (the draws PathIntegrator::Li makes, path.cc:42-125). */
-EXPORT void orc_li_mt(void* p, const double* org, const double* dir, uint32_t seed, int max_depth, int rr_depth,
-                      double* L_out) {
+/* Paths with spica's MT19937 Independent sampler, one seeded sampler per path, starting
+ * from given rays (the draws PathIntegrator::Li, or DirectLightingIntegrator::Li when
+ * `direct`, makes: path.cc:42-125, directlighting.cc:21-57).  The BxDFs' hidden streams
+ * start default-seeded at the first path and run on across the batch, as spica's
+ * thread_local ones do on one fresh thread. */
+EXPORT void orc_li_mt_batch(void* p, int64_t n, const double* org, const double* dir, const uint32_t* seeds,
+                            int max_depth, int rr_depth, int direct, double* L_out) {
     Scene* s = (Scene*)p;
-    MT mt;
-    mt_seed(&mt, seed);
-    Sampler sp = {SPK_SAMPLER_MT19937, seed, 0, 0, &mt};
-    Ray r = make_ray(v3(org[0], org[1], org[2]), v3(dir[0], dir[1], dir[2]), INFTY);
-    V3 L = Li_path(s, r, &sp, max_depth, rr_depth, NULL);
-    L_out[0] = L.x;
-    L_out[1] = L.y;
-    L_out[2] = L.z;
+    MT hmt[3];
+    Hidden hid;
+    hidden_init_mt(&hid, hmt);
+    for (int64_t i = 0; i < n; i++) {
+        MT mt;
+        mt_seed(&mt, seeds[i]);
+        Sampler sp = {SPK_SAMPLER_MT19937, seeds[i], 0, 0, &mt, &hid};
+        Ray r = make_ray(v3(org[3 * i], org[3 * i + 1], org[3 * i + 2]), v3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                         INFTY);
+        V3 L = direct ? Li_direct(s, r, &sp, max_depth, 0, 0, NULL) : Li_path(s, r, &sp, max_depth, rr_depth, NULL);
+        L_out[3 * i] = L.x;
+        L_out[3 * i + 1] = L.y;
+        L_out[3 * i + 2] = L.z;
+    }
 }
 
-/* The same for DirectLightingIntegrator::Li (directlighting.cc:21-57). */
+EXPORT void orc_li_mt(void* p, const double* org, const double* dir, uint32_t seed, int max_depth, int rr_depth,
+                      double* L_out) {
+    orc_li_mt_batch(p, 1, org, dir, &seed, max_depth, rr_depth, 0, L_out);
+}
+
 EXPORT void orc_li_mt_direct(void* p, const double* org, const double* dir, uint32_t seed, int max_depth,
                              double* L_out) {
-    Scene* s = (Scene*)p;
-    MT mt;
-    mt_seed(&mt, seed);
-    Sampler sp = {SPK_SAMPLER_MT19937, seed, 0, 0, &mt};
-    Ray r = make_ray(v3(org[0], org[1], org[2]), v3(dir[0], dir[1], dir[2]), INFTY);
-    V3 L = Li_direct(s, r, &sp, max_depth, 0, 0, NULL);
-    L_out[0] = L.x;
-    L_out[1] = L.y;
-    L_out[2] = L.z;
+    orc_li_mt_batch(p, 1, org, dir, &seed, max_depth, 3, 1, L_out);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1997,7 +2378,7 @@ typedef struct {
     PathStats stats;
 } RenderJob;
 
-static void render_pixel(RenderJob* j, int64_t k, PathStats* ps) {
+static void render_pixel(RenderJob* j, int64_t k, PathStats* ps, Hidden* hid) {
     const Cam* c = &j->cam;
     int64_t film = j->subset ? j->subset[k] : k;
     int fy = (int)(film / c->W), fx = (int)(film % c->W);
@@ -2006,7 +2387,7 @@ static void render_pixel(RenderJob* j, int64_t k, PathStats* ps) {
     MT mt;
     for (int si = 0; si < j->rd.spp; si++) {
         uint32_t sample = (uint32_t)(j->rd.first_sample + si);
-        Sampler sp = {j->rd.sampler, j->rd.seed, pix, sample, NULL};
+        Sampler sp = {j->rd.sampler, j->rd.seed, pix, sample, NULL, hid};
         if (j->rd.sampler == SPK_SAMPLER_MT19937) {
             uint32_t h[4] = {pix, sample, j->rd.seed, 0x2545f491u};
             pcg4d(h);
@@ -2041,6 +2422,9 @@ static void render_pixel(RenderJob* j, int64_t k, PathStats* ps) {
 static void* render_worker(void* arg) {
     RenderJob* j = (RenderJob*)arg;
     PathStats ps = {0, 0, 0};
+    MT hmt[3];
+    Hidden hid = {0, {NULL, NULL, NULL}, 0.0};
+    if (j->rd.sampler == SPK_SAMPLER_MT19937) hidden_init_mt(&hid, hmt);
     for (;;) {
         pthread_mutex_lock(&j->mu);
         int64_t k0 = j->next;
@@ -2048,7 +2432,7 @@ static void* render_worker(void* arg) {
         pthread_mutex_unlock(&j->mu);
         if (k0 >= j->npix) break;
         int64_t k1 = k0 + 16 < j->npix ? k0 + 16 : j->npix;
-        for (int64_t k = k0; k < k1; k++) render_pixel(j, k, &ps);
+        for (int64_t k = k0; k < k1; k++) render_pixel(j, k, &ps, &hid);
     }
     pthread_mutex_lock(&j->mu);
     j->stats.rays += ps.rays;

@@ -13,6 +13,12 @@ SPK_MAT_DIFFUSE = 1
 SPK_MAT_CONDUCTOR = 2
 SPK_MAT_DIELECTRIC = 3
 SPK_MAT_ROUGHCONDUCTOR = 4
+SPK_MAT_PLASTIC = 5
+SPK_MAT_ROUGHDIELECTRIC = 6
+SPK_MAT_ROUGHPLASTIC = 7
+
+SPK_MICROFACET_GGX = 0
+SPK_MICROFACET_BECKMANN = 1
 
 SPK_SAMPLER_INDEPENDENT = 0
 SPK_SAMPLER_SOBOL = 1
@@ -39,13 +45,14 @@ c_u32_p = ctypes.POINTER(ctypes.c_uint32)
 class SpkMaterial(ctypes.Structure):
     _fields_ = [
         ("type", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("distribution", ctypes.c_int32),
         ("reflectance", ctypes.c_double * 3),
         ("transmittance", ctypes.c_double * 3),
         ("eta", ctypes.c_double * 3),
         ("k", ctypes.c_double * 3),
         ("int_ior", ctypes.c_double),
         ("alpha", ctypes.c_double),
+        ("specular", ctypes.c_double * 3),
     ]
 
 

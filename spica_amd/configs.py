@@ -24,7 +24,8 @@ import math
 import numpy as np
 
 from . import _abi
-from .scene import GOLD, SILVER, Camera, Scene, box, conductor, dielectric, diffuse, look_at, quad, roughconductor, unit_sphere
+from .scene import (GOLD, SILVER, Camera, Scene, box, conductor, dielectric, diffuse, look_at, plastic, quad,
+                    roughconductor, roughdielectric, roughplastic, unit_sphere)
 
 
 class Workload:
@@ -78,7 +79,8 @@ def cornell_box(width=256, height=256, spp=64, max_depth=8, light=(17.0, 12.0, 4
 
 
 def _sphere_materials(rng, mix):
-    """mix: list of (probability, kind) with kind in lambert/mirror/dielectric/ggx(lo,hi)."""
+    """mix: list of (probability, kind) with kind in lambert/mirror/dielectric/ggx(lo,hi), and
+    beckmann(lo,hi) (rough gold), plastic, roughdielectric(lo,hi,dist), roughplastic(lo,hi,dist)."""
     u = rng.random()
     acc = 0.0
     for p, kind in mix:
@@ -91,10 +93,24 @@ def _sphere_materials(rng, mix):
         return conductor(**SILVER)
     if kind[0] == "dielectric":
         return dielectric(1.5)
+    if kind[0] == "beckmann":
+        return roughconductor(alpha=rng.uniform(kind[1], kind[2]), distribution="beckmann", **GOLD)
+    if kind[0] == "plastic":
+        return plastic(rng.uniform(0.1, 0.9, size=3), (1.0, 1.0, 1.0), 1.5)
+    if kind[0] == "roughdielectric":
+        return roughdielectric(alpha=rng.uniform(kind[1], kind[2]), int_ior=1.5, distribution=kind[3])
+    if kind[0] == "roughplastic":
+        return roughplastic(alpha=rng.uniform(kind[1], kind[2]), diffuse_reflectance=rng.uniform(0.1, 0.9, size=3),
+                            distribution=kind[3])
     return roughconductor(alpha=rng.uniform(kind[1], kind[2]), **GOLD)
 
 
 C1_MIX = [(0.4, ("lambert",)), (0.2, ("mirror",)), (0.2, ("dielectric",)), (0.2, ("ggx", 0.05, 0.5))]
+# every BSDF kind, both microfacet distributions
+WIDE_MIX = [(0.15, ("lambert",)), (0.1, ("mirror",)), (0.1, ("dielectric",)), (0.1, ("ggx", 0.05, 0.5)),
+            (0.1, ("beckmann", 0.05, 0.5)), (0.1, ("plastic",)), (0.1, ("roughdielectric", 0.05, 0.4, "beckmann")),
+            (0.05, ("roughdielectric", 0.05, 0.4, "ggx")), (0.1, ("roughplastic", 0.05, 0.4, "beckmann")),
+            (0.1, ("roughplastic", 0.05, 0.4, "ggx"))]
 C3_MIX = [(0.6, ("ggx", 0.1, 0.3)), (0.25, ("lambert",)), (0.1, ("mirror",)), (0.05, ("dielectric",))]
 
 

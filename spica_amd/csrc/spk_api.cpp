@@ -355,8 +355,14 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     }
     for (int m = 0; m < d->num_materials; m++) {
         int ty = d->materials[m].type;
-        if (ty < SPK_MAT_DIFFUSE || ty > SPK_MAT_ROUGHCONDUCTOR) {
+        if (ty < SPK_MAT_DIFFUSE || ty > SPK_MAT_ROUGHPLASTIC) {
             set_error("spk_scene_create: material %d has unknown type %d", m, ty);
+            return SPK_ERR_INVALID;
+        }
+        const int dist = d->materials[m].distribution;
+        if ((ty == SPK_MAT_ROUGHCONDUCTOR || ty == SPK_MAT_ROUGHDIELECTRIC || ty == SPK_MAT_ROUGHPLASTIC) &&
+            dist != SPK_MICROFACET_GGX && dist != SPK_MICROFACET_BECKMANN) {
+            set_error("spk_scene_create: material %d has unknown microfacet distribution %d", m, dist);
             return SPK_ERR_INVALID;
         }
     }
@@ -421,7 +427,8 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
         auto cl = [](double x) { return std::max(0.0, std::min(x, 1.0e32)); };
         int lobe = LOBE_NONE;
         D3 R = d3(1, 1, 1), T = d3(0, 0, 0);
-        double etaB = 1.0, alpha = 0.0;
+        double etaB = 1.0, alpha = 0.0, specW = 0.0;
+        const int dist = M.distribution == SPK_MICROFACET_BECKMANN ? 1 : 0;
         switch (M.type) {
         case SPK_MAT_DIFFUSE:
             R = d3(cl(M.reflectance[0]), cl(M.reflectance[1]), cl(M.reflectance[2]));
@@ -440,13 +447,41 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
             alpha = dgray(M.alpha, M.alpha, M.alpha);
             lobe = alpha == 0.0 ? LOBE_SPECREFL : LOBE_MICROREFL;
             break;
+        case SPK_MAT_PLASTIC:  // PlasticBRDF(Kd, Ks, 1, gray(intIOR)), no clamping
+            R = d3(M.reflectance[0], M.reflectance[1], M.reflectance[2]);
+            T = d3(M.specular[0], M.specular[1], M.specular[2]);
+            etaB = dgray(M.int_ior, M.int_ior, M.int_ior);
+            specW = dgray(T.x, T.y, T.z) / (dgray(R.x, R.y, R.z) + dgray(T.x, T.y, T.z));
+            lobe = LOBE_PLASTIC;
+            break;
+        case SPK_MAT_ROUGHDIELECTRIC:
+            R = d3(cl(M.reflectance[0]), cl(M.reflectance[1]), cl(M.reflectance[2]));
+            T = d3(cl(M.transmittance[0]), cl(M.transmittance[1]), cl(M.transmittance[2]));
+            etaB = dgray(M.int_ior, M.int_ior, M.int_ior);
+            alpha = dgray(M.alpha, M.alpha, M.alpha);
+            if (R.x == 0 && R.y == 0 && R.z == 0 && T.x == 0 && T.y == 0 && T.z == 0) lobe = LOBE_NONE;
+            else lobe = alpha == 0.0 ? LOBE_FRESNELSPEC : LOBE_MICROTRANS;
+            break;
+        case SPK_MAT_ROUGHPLASTIC:
+            R = d3(M.reflectance[0], M.reflectance[1], M.reflectance[2]);
+            T = d3(M.specular[0], M.specular[1], M.specular[2]);
+            if (T.x == 0 && T.y == 0 && T.z == 0) {  // LambertianReflection(kd), unclamped
+                lobe = LOBE_LAMBERT;
+                break;
+            }
+            etaB = dgray(M.int_ior, M.int_ior, M.int_ior);
+            alpha = std::max(0.001, dgray(M.alpha, M.alpha, M.alpha));
+            specW = dgray(T.x, T.y, T.z) / (dgray(R.x, R.y, R.z) + dgray(T.x, T.y, T.z));
+            lobe = LOBE_ROUGHPLASTIC;
+            break;
         }
+        const int lobe_bits = lobe | (dist << 8);
         float lw;
-        memcpy(&lw, &lobe, 4);
+        memcpy(&lw, &lobe_bits, 4);
         md.a = f4(R, lw);
         md.b = f4(T, (float)etaB);
         md.c = f4(d3(M.eta[0], M.eta[1], M.eta[2]), (float)alpha);
-        md.d = f4(d3(M.k[0], M.k[1], M.k[2]), 0.0f);
+        md.d = f4(d3(M.k[0], M.k[1], M.k[2]), (float)specW);
         mats[(size_t)m] = md;
     }
 
@@ -573,6 +608,12 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     sd.num_areas = d->num_area_lights;
     sd.num_lights = d->num_area_lights + (env.present ? 1 : 0);
     sd.has_bvh = n > 0 ? 1 : 0;
+    sd.wide = 0;
+    for (int m = 0; m < d->num_materials; m++) {
+        const spk_material& M = d->materials[m];
+        if (M.type >= SPK_MAT_PLASTIC || (M.type == SPK_MAT_ROUGHCONDUCTOR && M.distribution == SPK_MICROFACET_BECKMANN))
+            sd.wide = 1;
+    }
     sd.env = env;
 
     s->spill_cap = std::max(8, bvh.max_depth + 2);
@@ -731,6 +772,7 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     c.sampler = desc->sampler;
     c.seed = desc->seed;
     c.integrator = desc->integrator;
+    c.wide = s->dev.wide;
 
     // Buffers.
     int rc;
@@ -1279,6 +1321,7 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
     s0->spill_cap = s0_cap;
     if (rc) return rc;
     spk_render* r = b->r;
+    for (int i = 0; i < d->num_images; i++) r->cfg.wide |= b->scenes[i]->dev.wide;
     const uint32_t cap = r->capacity;
     for (int i = 0; i < d->num_images; i++) {
         const spk_camera_desc& c = d->images[i].camera;

@@ -13,10 +13,21 @@
 namespace spk {
 
 // Material lobes after spica's setScatterFuncs (bsdfs/diffuse.cc:23-32,
-// conductor.cc:28-38, dielectric.cc:30-42, roughconductor.cc:38-75).
-enum Lobe : int { LOBE_NONE = 0, LOBE_LAMBERT = 1, LOBE_SPECREFL = 2, LOBE_FRESNELSPEC = 3, LOBE_MICROREFL = 4 };
+// conductor.cc:28-38, dielectric.cc:30-42, roughconductor.cc:38-75, plastic.cc:111-121,
+// roughdielectric.cc:42-82, roughplastic.cc:128-158).
+enum Lobe : int {
+    LOBE_NONE = 0,
+    LOBE_LAMBERT = 1,
+    LOBE_SPECREFL = 2,
+    LOBE_FRESNELSPEC = 3,
+    LOBE_MICROREFL = 4,
+    LOBE_PLASTIC = 5,      // PlasticBRDF
+    LOBE_MICROTRANS = 6,   // MicrofacetTransmission
+    LOBE_ROUGHPLASTIC = 7  // RoughPlasticBRDF
+};
 
-// a = (R, lobe), b = (T, etaB), c = (eta, alpha), d = (k, 0)
+// a = (R, lobe | distribution << 8), b = (T, etaB), c = (eta, alpha), d = (k, specular
+// sampling weight).  R is Kd and T is Ks for the plastics.
 struct MatDev {
     float4 a, b, c, d;
 };
@@ -57,7 +68,7 @@ struct SceneDev {
     int num_areas;
     int num_lights;         // areas + env
     int has_bvh;
-    int pad;
+    int wide;               // holds Beckmann or plastic / glossy-transmission materials
     EnvDev env;
 };
 
@@ -116,6 +127,7 @@ struct ConfigDev {
     int sampler;
     uint32_t seed;
     int integrator;         // SPK_INTEGRATOR_PATH / SPK_INTEGRATOR_DIRECT
+    int wide;               // some scene needs the wide-material shading kernel
 };
 
 // Device-side per-pass state: queue lengths and work counters per bounce.

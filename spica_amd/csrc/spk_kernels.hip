@@ -655,13 +655,19 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, RenderBufs rb, in
 enum { BX_REFLECTION = 1, BX_TRANSMISSION = 2, BX_DIFFUSE = 4, BX_GLOSSY = 8, BX_SPECULAR = 16, BX_ALL = 31 };
 
 struct Bsdf {
-    int lobe, type;
-    float3 R, T, eta, k;
-    float etaB, alpha;
+    int lobe, type, dist;
+    float3 R, T, eta, k;  // T is Ks for the plastics
+    float etaB, alpha, specW;
     float3 t, b, n;  // frame
 };
 
+template <bool W>
 __device__ __forceinline__ int lobe_type(int lobe) {
+    if (W) {
+        if (lobe == LOBE_PLASTIC) return BX_REFLECTION | BX_SPECULAR | BX_DIFFUSE;
+        if (lobe == LOBE_MICROTRANS) return BX_REFLECTION | BX_TRANSMISSION | BX_GLOSSY;
+        if (lobe == LOBE_ROUGHPLASTIC) return BX_REFLECTION | BX_DIFFUSE | BX_GLOSSY;
+    }
     switch (lobe) {
     case LOBE_LAMBERT: return BX_REFLECTION | BX_DIFFUSE;
     case LOBE_SPECREFL: return BX_REFLECTION | BX_SPECULAR;
@@ -792,6 +798,139 @@ __device__ __forceinline__ float3 ggx_sample_wh(float alpha, float3 wo, float u0
     float3 wh = normalize(mk(-sx, -sy, 1.0f));
     return flip ? neg(wh) : wh;
 }
+// math.h:14-65: erfinv (Giles 2011, single-precision coefficients) and erf
+// (Abramowitz & Stegun 7.1.26), as spica's Beckmann sampling uses them.
+__device__ __forceinline__ float erfinv_s(float x) {
+    x = clampf(x, -0.99999f, 0.99999f);
+    float w = -logf((1.0f - x) * (1.0f + x)), p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = 3.43273939e-07f + p * w;
+        p = -3.5233877e-06f + p * w;
+        p = -4.39150654e-06f + p * w;
+        p = 0.00021858087f + p * w;
+        p = -0.00125372503f + p * w;
+        p = -0.00417768164f + p * w;
+        p = 0.246640727f + p * w;
+        p = 1.50140941f + p * w;
+    } else {
+        w = sqrtf(w) - 3.0f;
+        p = -0.000200214257f;
+        p = 0.000100950558f + p * w;
+        p = 0.00134934322f + p * w;
+        p = -0.00367342844f + p * w;
+        p = 0.00573950773f + p * w;
+        p = -0.0076224613f + p * w;
+        p = 0.00943887047f + p * w;
+        p = 1.00167406f + p * w;
+        p = 2.83297682f + p * w;
+    }
+    return p * x;
+}
+__device__ __forceinline__ float erf_s(float x) {
+    const float sign = x < 0.0f ? -1.0f : 1.0f;
+    x = fabsf(x);
+    const float t = 1.0f / (1.0f + 0.3275911f * x);
+    const float y = 1.0f - (((((1.061405429f * t - 1.453152027f) * t) + 1.421413741f) * t - 0.284496736f) * t +
+                            0.254829592f) * t * expf(-x * x);
+    return sign * y;
+}
+// microfacet.cc:277-288
+__device__ __forceinline__ float bk_D(float alpha, float3 wh) {
+    float c2 = wh.z * wh.z;
+    float t2 = sin2t(wh) / (c2 + kEPS);
+    if (__builtin_isinf(t2)) return 0.0f;
+    float cp = cos_phi(wh), sp = sin_phi(wh);
+    float ab2 = cp * cp / (alpha * alpha) + sp * sp / (alpha * alpha);
+    return expf(-t2 * ab2) / (kPI * alpha * alpha * c2 * c2);
+}
+// microfacet.cc:342-355 (Walter et al. 2007's rational fit of the Smith lambda)
+__device__ __forceinline__ float bk_lambda(float alpha, float3 w) {
+    float at = fabsf(sqrtf(sin2t(w)) / (w.z + kEPS));
+    if (__builtin_isinf(at)) return 0.0f;
+    float cp = cos_phi(w), sp = sin_phi(w);
+    float al = sqrtf(cp * cp * alpha * alpha + sp * sp * alpha * alpha);
+    float a = 1.0f / (al * at);
+    if (a >= 1.6f) return 0.0f;
+    return (1.0f - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+}
+// microfacet.cc:80-131: visible slopes by Jakob 2014's bisection-Newton solve (16 steps at most)
+__device__ void bk_sample11(float cosI, float U1, float U2, float& sx, float& sy) {
+    if (cosI > 0.9999f) {
+        float r = sqrtf(-logf(1.0f - U1));
+        sx = r * cosf(2.0f * kPI * U2);
+        sy = r * sinf(2.0f * kPI * U2);
+        return;
+    }
+    const float sinI = sqrtf(fmaxf(0.0f, 1.0f - cosI * cosI));
+    const float tanI = sinI / cosI, cotI = 1.0f / tanI;
+    float a = -1.0f, c = erf_s(cotI);
+    const float x0 = fmaxf(U1, 1.0e-6f);
+    const float th = acosf(cosI);
+    const float fit = 1.0f + th * (-0.876f + th * (0.4265f - 0.0594f * th));
+    float bb = c - (1.0f + c) * powf(1.0f - x0, fit);
+    const float kSqrtPiInv = 0.5641895835477563f;  // 1 / sqrt(pi)
+    const float norm = 1.0f / (1.0f + c + kSqrtPiInv * tanI * expf(-cotI * cotI));
+    for (int it = 0; it < 16; it++) {
+        if (bb < a || c < bb) bb = 0.5f * (a + c);
+        const float xm = erfinv_s(bb);
+        const float value = norm * (1.0f + bb + kSqrtPiInv * tanI * expf(-xm * xm)) - x0;
+        const float deriv = norm * (1.0f - xm * tanI);
+        if (fabsf(value) < 1.0e-6f) break;
+        if (value > 0.0f) c = bb;
+        else a = bb;
+        bb -= value / deriv;
+    }
+    sx = erfinv_s(bb);
+    sy = erfinv_s(2.0f * fmaxf(U2, 1.0e-6f) - 1.0f);
+}
+__device__ __forceinline__ float3 bk_sample_wh(float alpha, float3 wo, float u0, float u1) {
+    bool flip = wo.z < 0.0f;
+    float3 wi = flip ? neg(wo) : wo;
+    float3 ws = normalize(mk(alpha * wi.x, alpha * wi.y, wi.z));
+    float sx, sy;
+    bk_sample11(ws.z, u0, u1, sx, sy);
+    float cp = cos_phi(ws), sp = sin_phi(ws);
+    float tmp = cp * sx - sp * sy;
+    sy = (sp * sx + cp * sy) * alpha;
+    sx = tmp * alpha;
+    float3 wh = normalize(mk(-sx, -sy, 1.0f));
+    return flip ? neg(wh) : wh;
+}
+// MicrofacetDistribution of the material's "distribution" (microfacet.cc:166-193)
+// W (wide): the kernel may meet Beckmann and the plastic / glossy-transmission lobes.
+// Scenes without them run kernels built with W = false, which leave that code out and
+// keep the registers of the four basic lobes.
+template <bool W>
+__device__ __forceinline__ float mf_D(const Bsdf& b, float3 wh) {
+    return (W && b.dist) ? bk_D(b.alpha, wh) : ggx_D(b.alpha, wh);
+}
+template <bool W>
+__device__ __forceinline__ float mf_lambda(const Bsdf& b, float3 w) {
+    return (W && b.dist) ? bk_lambda(b.alpha, w) : ggx_lambda(b.alpha, w);
+}
+template <bool W>
+__device__ __forceinline__ float mf_G(const Bsdf& b, float3 wo, float3 wi, float3 wh) {
+    if (wi.z * dot(wi, wh) <= 0.0f) return 0.0f;
+    if (wo.z * dot(wo, wh) <= 0.0f) return 0.0f;
+    return 1.0f / (1.0f + mf_lambda<W>(b, wo) + mf_lambda<W>(b, wi));
+}
+template <bool W>
+__device__ __forceinline__ float mf_pdf(const Bsdf& b, float3 wo, float3 wh) {
+    float g1 = wo.z * dot(wo, wh) <= 0.0f ? 0.0f : 1.0f / (1.0f + mf_lambda<W>(b, wo));
+    return mf_D<W>(b, wh) * g1 * fabsf(dot(wo, wh)) / fabsf(wo.z);
+}
+template <bool W>
+__device__ __forceinline__ float3 mf_sample_wh(const Bsdf& b, float3 wo, float u0, float u1) {
+    return (W && b.dist) ? bk_sample_wh(b.alpha, wo, u0, u1) : ggx_sample_wh(b.alpha, wo, u0, u1);
+}
+// fresnel.cc:87-95
+__device__ __forceinline__ float fr_diffuse_reflectance(float eta) {
+    if (eta >= 1.0f) return -1.4399f / (eta * eta) + 0.7099f / eta + 0.6681f + 0.0636f * eta;
+    return -0.4399f + 0.7099f / eta - 0.3319f / (eta * eta) + 0.0636f / (eta * eta * eta);
+}
+template <bool W>
 __device__ __forceinline__ float3 micro_f(const Bsdf& b, float3 wo, float3 wi) {
     float cO = fabsf(wo.z), cI = fabsf(wi.z);
     float3 wh = add(wi, wo);
@@ -799,7 +938,7 @@ __device__ __forceinline__ float3 micro_f(const Bsdf& b, float3 wo, float3 wi) {
     if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk(0, 0, 0);
     wh = normalize(wh);
     float3 F = fr_conductor(fabsf(dot(wi, wh)), b.eta, b.k);
-    float s = ggx_D(b.alpha, wh) * ggx_G(b.alpha, wo, wi, wh) / (4.0f * cI * cO);
+    float s = mf_D<W>(b, wh) * mf_G<W>(b, wo, wi, wh) / (4.0f * cI * cO);
     return mul(cmul(b.R, F), s);
 }
 __device__ __forceinline__ bool refract_v(float3 wi, float3 n, float eta, float3& wt) {
@@ -812,6 +951,156 @@ __device__ __forceinline__ bool refract_v(float3 wi, float3 n, float eta, float3
     return true;
 }
 
+// bxdf.cc:333-354: MicrofacetTransmission::f(wo, wi, wh)
+__device__ float3 mtrans_f_wh(const Bsdf& b, float3 wo, float3 wi, float3 wh) {
+    const float cO = wo.z, cI = wi.z;
+    if (cO == 0.0f || cI == 0.0f) return mk(0, 0, 0);
+    if (wo.z * dot(wo, wh) <= 0.0f || wi.z * dot(wi, wh) <= 0.0f) return mk(0, 0, 0);
+    const float F = fr_dielectric(dot(wo, wh), 1.0f, b.etaB);
+    if (wo.z * wi.z > 0.0f) return mul(b.R, F * mf_D<true>(b, wh) * mf_G<true>(b, wo, wi, wh) / (4.0f * cO * cI));
+    const float eta = dot(wo, wh) > 0.0f ? b.etaB : 1.0f / b.etaB;
+    const float factor = 1.0f / eta;
+    const float sd = dot(wo, wh) + eta * dot(wi, wh);
+    return mul(b.T, (1.0f - F) * fabsf(mf_D<true>(b, wh) * mf_G<true>(b, wo, wi, wh) * eta * eta * fabsf(dot(wi, wh)) *
+                                      fabsf(dot(wo, wh)) * factor * factor / (cI * cO * sd * sd)));
+}
+// bxdf.cc:408-423
+__device__ float mtrans_pdf_wh(const Bsdf& b, float3 wo, float3 wi, float3 wh) {
+    if (wo.z * dot(wo, wh) <= 0.0f || wi.z * dot(wi, wh) <= 0.0f) return 0.0f;
+    const float F = fr_dielectric(dot(wo, wh), 1.0f, b.etaB);
+    if (wo.z * wi.z > 0.0f) return F * mf_pdf<true>(b, wo, wh) / (4.0f * fabsf(dot(wo, wh)));
+    const float eta = dot(wo, wh) > 0.0f ? b.etaB : 1.0f / b.etaB;
+    const float sd = dot(wo, wh) + eta * dot(wi, wh);
+    return (1.0f - F) * mf_pdf<true>(b, wo, wh) * fabsf((eta * eta * dot(wi, wh)) / (sd * sd));
+}
+// bxdf.cc:315-331, 394-406: the half vector of a (wo, wi) pair on the +z side
+__device__ __forceinline__ float3 mtrans_wh(const Bsdf& b, float3 wo, float3 wi) {
+    float3 wh = wo.z * wi.z > 0.0f ? normalize(add(wo, wi))
+                                   : normalize(add(wo, mul(wi, wo.z > 0.0f ? b.etaB : 1.0f / b.etaB)));
+    return wh.z >= 0.0f ? wh : neg(wh);
+}
+// plastic.cc:28, roughplastic.cc:28: probability of the specular branch
+__device__ __forceinline__ float plastic_prob_spec(const Bsdf& b, float Fo) {
+    return (Fo * b.specW) / (Fo * b.specW + (1.0f - Fo) * (1.0f - b.specW));
+}
+// The plastics' diffuse term with the internal-scattering normalisation (plastic.cc:62-67)
+__device__ __forceinline__ float3 plastic_diffuse(const Bsdf& b, float Fo, float cosI) {
+    const float invEta = 1.0f / b.etaB;
+    const float Fi = fr_dielectric(cosI, 1.0f, b.etaB);
+    return mul(b.R, (1.0f - Fo) * (1.0f - Fi) * invEta * invEta * kINV_PI / (1.0f - fr_diffuse_reflectance(b.etaB)));
+}
+__device__ __forceinline__ float3 cosine_hemisphere(float u0, float u1) {
+    float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f, dx = 0.0f, dy = 0.0f;
+    if (ox != 0.0f || oy != 0.0f) {
+        float r, th;
+        if (fabsf(ox) > fabsf(oy)) {
+            r = ox;
+            th = kPI * (oy / ox) / 4.0f;
+        } else {
+            r = oy;
+            th = (kPI / 2.0f) - kPI * (ox / oy) / 4.0f;
+        }
+        dx = r * cosf(th);
+        dy = r * sinf(th);
+    }
+    return mk(dx, dy, sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy)));
+}
+
+// The plastics and the glossy transmission (wide kernels only).
+__device__ __forceinline__ float3 lobe_f_ext(const Bsdf& b, float3 wo, float3 wi) {
+    switch (b.lobe) {
+    case LOBE_PLASTIC: {  // plastic.cc:32-46 (the "specular" test compares wi with its own mirror image)
+        const float Fo = fr_dielectric(wo.z, 1.0f, b.etaB);
+        if (dot(wi, mk(-wi.x, -wi.y, wi.z)) > 1.0f - kEPS) return mul(b.T, Fo / fabsf(wi.z));
+        return plastic_diffuse(b, Fo, wi.z);
+    }
+    case LOBE_MICROTRANS:  // bxdf.cc:315-331
+        if (wo.z == 0.0f || wi.z == 0.0f) return mk(0, 0, 0);
+        return mtrans_f_wh(b, wo, wi, mtrans_wh(b, wo, wi));
+    case LOBE_ROUGHPLASTIC: {  // roughplastic.cc:35-47
+        if (wo.z == 0.0f || wi.z == 0.0f || !(wo.z * wi.z > 0.0f)) return mk(0, 0, 0);
+        const float3 wh = normalize(add(wi, wo));
+        const float F = fr_dielectric(dot(wo, wh), 1.0f, b.etaB);
+        return add(mul(b.T, F * mf_D<true>(b, wh) * mf_G<true>(b, wo, wi, wh) / (4.0f * wo.z * wi.z)),
+                   mul(b.R, (1.0f - F) * kINV_PI));
+    }
+    default:
+        return mk(0, 0, 0);
+    }
+}
+__device__ __forceinline__ float lobe_pdf_ext(const Bsdf& b, float3 wo, float3 wi) {
+    switch (b.lobe) {
+    case LOBE_PLASTIC: {  // plastic.cc:71-82
+        const float ps = plastic_prob_spec(b, fr_dielectric(wo.z, 1.0f, b.etaB));
+        if (dot(wi, mk(-wi.x, -wi.y, wi.z)) > 1.0f - kEPS) return ps;
+        return (1.0f - ps) * fabsf(wi.z) * kINV_PI;
+    }
+    case LOBE_MICROTRANS:  // bxdf.cc:394-406
+        return mtrans_pdf_wh(b, wo, wi, mtrans_wh(b, wo, wi));
+    case LOBE_ROUGHPLASTIC: {  // roughplastic.cc:88-97 (F-weighted, not the sampling weights)
+        if (!(wo.z * wi.z > 0.0f)) return 0.0f;
+        const float3 wh = normalize(add(wi, wo));
+        const float F = fr_dielectric(dot(wo, wh), 1.0f, b.etaB);
+        return F * mf_pdf<true>(b, wo, wh) / (4.0f * fabsf(dot(wo, wh))) + (1.0f - F) * fabsf(wi.z) * kINV_PI;
+    }
+    default:
+        return 0.0f;
+    }
+}
+__device__ __forceinline__ float3 lobe_sample_ext(const Bsdf& b, float3 wo, float3& wi, float u0, float u1, float& pdf,
+                                               int& stype, float uh) {
+    switch (b.lobe) {
+    case LOBE_PLASTIC: {  // plastic.cc:48-69 (no hemisphere flip of the diffuse sample)
+        const float Fo = fr_dielectric(wo.z, 1.0f, b.etaB);
+        const float ps = plastic_prob_spec(b, Fo);
+        if (uh < ps) {
+            wi = mk(-wo.x, -wo.y, wo.z);
+            pdf = ps;
+            return mul(b.T, Fo / fabsf(wi.z));
+        }
+        wi = cosine_hemisphere(u0, u1);
+        pdf = (1.0f - ps) * fabsf(wi.z) * kINV_PI;
+        return plastic_diffuse(b, Fo, wi.z);
+    }
+    case LOBE_MICROTRANS: {  // bxdf.cc:356-392
+        if (wo.z == 0.0f) return mk(0, 0, 0);
+        const float3 wh = mf_sample_wh<true>(b, wo, u0, u1);
+        const float cO = dot(wo, wh.z > 0.0f ? wh : neg(wh));
+        const float F = fr_dielectric(cO, 1.0f, b.etaB);
+        if (uh < F) {
+            wi = sub(mul(wh, 2.0f * dot(wh, wo)), wo);
+            if (!(wo.z * wi.z > 0.0f)) return mk(0, 0, 0);
+            stype = BX_REFLECTION | BX_GLOSSY;
+        } else {
+            if (!refract_v(wo, wh, cO > 0.0f ? 1.0f / b.etaB : b.etaB, wi)) return mk(0, 0, 0);
+            if (wo.z * wi.z > 0.0f) return mk(0, 0, 0);
+            stype = BX_TRANSMISSION | BX_GLOSSY;
+        }
+        const float3 whp = wh.z >= 0.0f ? wh : neg(wh);
+        pdf = mtrans_pdf_wh(b, wo, wi, whp);
+        return mtrans_f_wh(b, wo, wi, whp);
+    }
+    case LOBE_ROUGHPLASTIC: {  // roughplastic.cc:49-80 (the diffuse branch reuses u0, u1)
+        if (wo.z == 0.0f) return mk(0, 0, 0);
+        const float3 wh = mf_sample_wh<true>(b, wo, u0, u1);
+        const float Fo = fr_dielectric(dot(wo, wh), 1.0f, b.etaB);
+        const float ps = plastic_prob_spec(b, Fo);
+        if (uh < ps) {
+            wi = sub(mul(wh, 2.0f * dot(wh, wo)), wo);
+            if (!(wo.z * wi.z > 0.0f)) return mk(0, 0, 0);
+            pdf = ps * mf_pdf<true>(b, wo, wh) / (4.0f * fabsf(dot(wo, wh)));
+            return mul(b.T, Fo * mf_D<true>(b, wh) * mf_G<true>(b, wo, wi, wh) / (4.0f * wo.z * wi.z));
+        }
+        wi = cosine_hemisphere(u0, u1);
+        pdf = (1.0f - ps) * fabsf(wi.z) * kINV_PI;
+        return plastic_diffuse(b, Fo, wi.z);
+    }
+    default:
+        return mk(0, 0, 0);
+    }
+}
+
+template <bool W>
 __device__ float3 lobe_f(const Bsdf& b, float3 wo, float3 wi) {
     const float deltaEps = 1.0e-4f;
     switch (b.lobe) {
@@ -834,11 +1123,17 @@ __device__ float3 lobe_f(const Bsdf& b, float3 wo, float3 wi) {
         return mk(0, 0, 0);
     }
     case LOBE_MICROREFL:
-        return micro_f(b, wo, wi);
+        return micro_f<W>(b, wo, wi);
+    case LOBE_PLASTIC:
+    case LOBE_MICROTRANS:
+    case LOBE_ROUGHPLASTIC:
+        if (W) return lobe_f_ext(b, wo, wi);
+        return mk(0, 0, 0);
     default:
         return mk(0, 0, 0);
     }
 }
+template <bool W>
 __device__ float lobe_pdf(const Bsdf& b, float3 wo, float3 wi) {
     const float deltaEps = 1.0e-4f;
     switch (b.lobe) {
@@ -862,33 +1157,28 @@ __device__ float lobe_pdf(const Bsdf& b, float3 wo, float3 wi) {
     case LOBE_MICROREFL: {
         if (!(wo.z * wi.z > 0.0f)) return 0.0f;
         float3 wh = normalize(add(wo, wi));
-        return ggx_pdf(b.alpha, wo, wh) / (4.0f * dot(wo, wh));
+        return mf_pdf<W>(b, wo, wh) / (4.0f * dot(wo, wh));
     }
+    case LOBE_PLASTIC:
+    case LOBE_MICROTRANS:
+    case LOBE_ROUGHPLASTIC:
+        if (W) return lobe_pdf_ext(b, wo, wi);
+        return 0.0f;
     default:
         return 0.0f;
     }
 }
-// Returns f; pdf stays 0 on failure; stype receives the sampled component type.
-__device__ float3 lobe_sample(const Bsdf& b, float3 wo, float3& wi, float u0, float u1, float& pdf, int& stype) {
+// Returns f; pdf stays 0 on failure; stype receives the sampled component type.  uh is
+// the BxDF's hidden uniform: spica's MicrofacetTransmission, PlasticBRDF and
+// RoughPlasticBRDF draw it from a thread-local Random; here it is a sampler dimension.
+template <bool W>
+__device__ float3 lobe_sample(const Bsdf& b, float3 wo, float3& wi, float u0, float u1, float& pdf, int& stype, float uh) {
     switch (b.lobe) {
     case LOBE_LAMBERT: {
-        float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f, dx = 0.0f, dy = 0.0f;
-        if (ox != 0.0f || oy != 0.0f) {
-            float r, th;
-            if (fabsf(ox) > fabsf(oy)) {
-                r = ox;
-                th = kPI * (oy / ox) / 4.0f;
-            } else {
-                r = oy;
-                th = (kPI / 2.0f) - kPI * (ox / oy) / 4.0f;
-            }
-            dx = r * cosf(th);
-            dy = r * sinf(th);
-        }
-        wi = mk(dx, dy, sqrtf(fmaxf(0.0f, 1.0f - dx * dx - dy * dy)));
+        wi = cosine_hemisphere(u0, u1);
         if (wo.z < 0.0f) wi.z *= -1.0f;
-        pdf = lobe_pdf(b, wo, wi);
-        return lobe_f(b, wo, wi);
+        pdf = lobe_pdf<W>(b, wo, wi);
+        return lobe_f<W>(b, wo, wi);
     }
     case LOBE_SPECREFL: {
         wi = mk(-wo.x, -wo.y, wo.z);
@@ -914,12 +1204,17 @@ __device__ float3 lobe_sample(const Bsdf& b, float3 wo, float3& wi, float u0, fl
     }
     case LOBE_MICROREFL: {
         if (wo.z == 0.0f) return mk(0, 0, 0);
-        float3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+        float3 wh = mf_sample_wh<W>(b, wo, u0, u1);
         wi = sub(mul(wh, 2.0f * dot(wh, wo)), wo);
         if (!(wo.z * wi.z > 0.0f)) return mk(0, 0, 0);
-        pdf = ggx_pdf(b.alpha, wo, wh) / (4.0f * dot(wo, wh));
-        return micro_f(b, wo, wi);
+        pdf = mf_pdf<W>(b, wo, wh) / (4.0f * dot(wo, wh));
+        return micro_f<W>(b, wo, wi);
     }
+    case LOBE_PLASTIC:
+    case LOBE_MICROTRANS:
+    case LOBE_ROUGHPLASTIC:
+        if (W) return lobe_sample_ext(b, wo, wi, u0, u1, pdf, stype, uh);
+        return mk(0, 0, 0);
     default:
         return mk(0, 0, 0);
     }
@@ -928,24 +1223,32 @@ __device__ __forceinline__ float3 to_local(const Bsdf& b, float3 v) { return mk(
 __device__ __forceinline__ float3 to_world(const Bsdf& b, float3 v) {
     return add(add(mul(b.t, v.x), mul(b.b, v.y)), mul(b.n, v.z));
 }
+// numComponents(All & ~Specular) > 0 (bsdf.cc:28-36, subset rule): the path's NEE gate
 __device__ __forceinline__ bool bsdf_has_nonspecular(const Bsdf& b) {
-    return b.lobe == LOBE_LAMBERT || b.lobe == LOBE_MICROREFL;
+    return b.lobe != LOBE_NONE && (b.type & BX_SPECULAR) == 0;
 }
+// BxDFs whose sample() draws a hidden uniform (only in wide kernels)
+template <bool W>
+__device__ __forceinline__ bool lobe_has_hidden(const Bsdf& b) { return W && b.lobe >= LOBE_PLASTIC; }
 // bsdf.cc:49-63
+template <bool W>
 __device__ float3 bsdf_f(const Bsdf& b, float3 woW, float3 wiW, int mask) {
     if (b.lobe == LOBE_NONE) return mk(0, 0, 0);
     bool reflect = dot(wiW, b.n) * dot(woW, b.n) > 0.0f;
     if ((b.type & mask) != 0 && ((reflect && (b.type & BX_REFLECTION)) || (!reflect && (b.type & BX_TRANSMISSION))))
-        return lobe_f(b, to_local(b, woW), to_local(b, wiW));
+        return lobe_f<W>(b, to_local(b, woW), to_local(b, wiW));
     return mk(0, 0, 0);
 }
 // bsdf.cc:124-136
+template <bool W>
 __device__ float bsdf_pdf(const Bsdf& b, float3 woW, float3 wiW, int mask) {
     if (b.lobe == LOBE_NONE || (b.type & mask) == 0) return 0.0f;
-    return lobe_pdf(b, to_local(b, woW), to_local(b, wiW));
+    return lobe_pdf<W>(b, to_local(b, woW), to_local(b, wiW));
 }
 // bsdf.cc:65-122 (one BxDF)
-__device__ float3 bsdf_sample(const Bsdf& b, float3 woW, float3& wiW, float u0, float u1, float& pdf, int mask, int& stype) {
+template <bool W>
+__device__ float3 bsdf_sample(const Bsdf& b, float3 woW, float3& wiW, float u0, float u1, float& pdf, int mask, int& stype,
+                              float uh) {
     pdf = 0.0f;
     stype = 0;
     if (b.lobe == LOBE_NONE || (b.type & mask) != b.type) return mk(0, 0, 0);
@@ -953,7 +1256,7 @@ __device__ float3 bsdf_sample(const Bsdf& b, float3 woW, float3& wiW, float u0, 
     if (wo.z == 0.0f) return mk(0, 0, 0);
     stype = b.type;
     float3 wi = mk(0, 0, 1);
-    float3 f = lobe_sample(b, wo, wi, u0, u1, pdf, stype);
+    float3 f = lobe_sample<W>(b, wo, wi, u0, u1, pdf, stype, uh);
     if (pdf == 0.0f) {
         stype = 0;
         return mk(0, 0, 0);
@@ -1065,10 +1368,16 @@ __device__ float area_pdf(const LightDev& L, float3 p, float3 ng, float3 wi) {
 __device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
 
 // Material lobe and parameters of a surface hit (bsdf.cc:14-21).
+// Scenes that reach kernels built with W = false hold no Beckmann material, so the lobe
+// word there is the lobe itself.
+template <bool W>
 __device__ __forceinline__ void bsdf_material(const SceneDev& sc, uint32_t mbits, Bsdf& b) {
     const MatDev M = sc.mats[mbits & 0x7fffffffu];
-    b.lobe = __float_as_int(M.a.w);
-    b.type = lobe_type(b.lobe);
+    const int lb = __float_as_int(M.a.w);
+    b.lobe = W ? (lb & 0xff) : lb;
+    b.dist = W ? (lb >> 8) : 0;
+    b.specW = M.d.w;
+    b.type = lobe_type<W>(b.lobe);
     b.R = xyz(M.a);
     b.T = xyz(M.b);
     b.etaB = M.b.w;
@@ -1094,13 +1403,22 @@ __device__ __forceinline__ void bsdf_frame(float3 ng, bool smooth, float3 ns, Bs
     }
 }
 
-// Vertices whose continuation the learned proposal samples: the non-specular lobes.
+// The hidden uniform of a path's BSDF sample at bounce `depth`: which = 0 for the NEE
+// BSDF sample, 1 for the continuation; past the path's 8 dimensions per bounce and the
+// learned proposal's block.
+__device__ __forceinline__ uint32_t hidden_dim_path(int depth, int which) {
+    return 4u + 8u * 128u + 2u * (uint32_t)depth + (uint32_t)which;
+}
+
+// Vertices whose continuation the learned proposal samples: Lambert and GGX reflection
+// (the guide kernel is built for the four basic lobes; the plastics' sample() does not
+// even agree with their f() and pdf(), roughplastic.cc:49-97).
 __device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
-    return (b.lobe == LOBE_LAMBERT && !black(b.R)) || b.lobe == LOBE_MICROREFL;
+    return (b.lobe == LOBE_LAMBERT && !black(b.R)) || (b.lobe == LOBE_MICROREFL && b.dist == 0);
 }
 
 // ---- shade ---------------------------------------------------------------------------------------
-template <int BLOCK, bool GUIDED, bool BATCH>
+template <int BLOCK, bool GUIDED, bool BATCH, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                  int depth) {
     PassState* ps = rb.ps;
@@ -1157,7 +1475,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                 if (direct || depth < max_depth) {
                     // BSDF frame (bsdf.cc:14-21 from triangle.cc:127-155)
                     Bsdf b;
-                    bsdf_material(sc, mbits, b);
+                    bsdf_material<WIDE>(sc, mbits, b);
                     float3 ns = ng;
                     if (mbits >> 31) {
                         const float4* tn = sc.tri_n + 3 * k;
@@ -1167,7 +1485,9 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                     bsdf_frame(ng, (mbits >> 31) != 0u, ns, b);
                     const float3 wo = neg(rd);
                     // NEE: uniformSampleOneLight + estimateDirectLight (mis.cc:21-136)
-                    if (bsdf_has_nonspecular(b) && sc.num_lights > 0) {
+                    // the path skips NEE at BSDFs without a non-specular component (path.cc:77);
+                    // directlighting always calls it (directlighting.cc:47-49)
+                    if ((direct ? b.lobe != LOBE_NONE : bsdf_has_nonspecular(b)) && sc.num_lights > 0) {
                         const int mask = BX_ALL & ~BX_SPECULAR;
                         float upick = sample_1d(kind, seed, pixel, sample, dim_base + 0);
                         int li = min((int)(upick * nl), sc.num_lights - 1);
@@ -1195,8 +1515,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                             lp = add(p, mul(wi, 2.0f * sc.env.radius));
                         }
                         if (lightPdf > 0.0f && !black(Li)) {
-                            float3 f = mul(bsdf_f(b, wo, wi, mask), fabsf(dot(wi, b.n)));
-                            float bsdfPdf = bsdf_pdf(b, wo, wi, mask);
+                            float3 f = mul(bsdf_f<WIDE>(b, wo, wi, mask), fabsf(dot(wi, b.n)));
+                            float bsdfPdf = bsdf_pdf<WIDE>(b, wo, wi, mask);
                             if (!black(f)) {
                                 // spawnRayTo(Interaction) (interaction.cc:88-93)
                                 float3 so = offset_origin(p, ng, sub(lp, p));
@@ -1214,7 +1534,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                         float bp = 0.0f;
                         int st = 0;
                         float3 wi2 = mk(0, 0, 1);
-                        float3 f2 = bsdf_sample(b, wo, wi2, us.x, us.y, bp, mask, st);
+                        const float uh2 = lobe_has_hidden<WIDE>(b) ? sample_1d(kind, seed, pixel, sample,
+                                                                         direct ? dim_base + 9 : hidden_dim_path(depth, 0))
+                                                             : 0.5f;
+                        float3 f2 = bsdf_sample<WIDE>(b, wo, wi2, us.x, us.y, bp, mask, st, uh2);
                         f2 = mul(f2, fabsf(dot(wi2, b.n)));
                         if (!black(f2) && bp > 0.0f) {
                             float lp2 = is_env ? env_pdf(sc.env, wi2) : area_pdf(sc.lights[li], p, ng, wi2);
@@ -1234,14 +1557,15 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                         // specularReflect (integrator.cc:112-130) while depth + 1 < maxDepth, where
                         // spica's depth grows by 2 per specular bounce (Li -> specularReflect(depth + 1)
                         // -> Li(depth + 2)); weighted by |wi . n_geometric| / pdf, no roulette.
-                        // specularTransmit never finds a component here: FresnelSpecular (R|T|S) is
-                        // no subset of T|S, and no other lobe transmits.
+                        // Only the mirror's type (R|S) is a subset of R|S; specularTransmit never finds a
+                        // component: no lobe's type is a subset of T|S (FresnelSpecular is R|T|S,
+                        // MicrofacetTransmission R|T|G).
                         if (2 * depth + 1 < max_depth) {
                             float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
                             float pdf = 0.0f;
                             int st = 0;
                             float3 wi = mk(0, 0, 1);
-                            float3 f = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf, BX_REFLECTION | BX_SPECULAR, st);
+                            float3 f = bsdf_sample<WIDE>(b, wo, wi, uc.x, uc.y, pdf, BX_REFLECTION | BX_SPECULAR, st, 0.5f);
                             const float cg = fabsf(dot(wi, ng));
                             if (pdf > 0.0f && !black(f) && cg != 0.0f) {
                                 pushA = true;
@@ -1262,7 +1586,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                     float pdf = 0.0f;
                     int st = 0;
                     float3 wi = mk(0, 0, 1);
-                    float3 f = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf, BX_ALL, st);
+                    const float uh = lobe_has_hidden<WIDE>(b) ? sample_1d(kind, seed, pixel, sample, hidden_dim_path(depth, 1)) : 0.5f;
+                    float3 f = bsdf_sample<WIDE>(b, wo, wi, uc.x, uc.y, pdf, BX_ALL, st, uh);
                     if (!black(f) && pdf != 0.0f) {
                         float3 nb = cmul(beta, mul(f, fabsf(dot(wi, b.n)) / pdf));
                         bool cont = true;
@@ -1472,7 +1797,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             const uint32_t mbits = __float_as_uint(NG.w);
             const float3 ng = xyz(NG), p = xyz(G0), wo = xyz(G1);
             Bsdf b;
-            bsdf_material(sc, mbits, b);
+            bsdf_material<false>(sc, mbits, b);
             bsdf_frame(ng, (mbits >> 31) != 0u, xyz(G3), b);
             const float2 uc = sample_2d(kind, seed, pixel, sample, dim_base + 5);
             const float us = sample_1d(kind, seed, pixel, sample, guide_dim(depth));
@@ -1483,12 +1808,12 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             } else {
                 float pdf_b = 0.0f;
                 int st = 0;
-                float3 fb = bsdf_sample(b, wo, wi, uc.x, uc.y, pdf_b, BX_ALL, st);
+                float3 fb = bsdf_sample<false>(b, wo, wi, uc.x, uc.y, pdf_b, BX_ALL, st, 0.5f);
                 ok = pdf_b > 0.0f && !black(fb);
             }
             if (ok) {
-                const float3 f = bsdf_f(b, wo, wi, BX_ALL);
-                const float pdf = (1.0f - net.frac) * bsdf_pdf(b, wo, wi, BX_ALL) +
+                const float3 f = bsdf_f<false>(b, wo, wi, BX_ALL);
+                const float pdf = (1.0f - net.frac) * bsdf_pdf<false>(b, wo, wi, BX_ALL) +
                                   net.frac * __expf(mix_log_pdf(mix, wi.x, wi.y, wi.z));
                 if (!black(f) && pdf > 0.0f) {
                     float3 nb = cmul(beta, mul(f, fabsf(dot(wi, b.n)) / pdf));
@@ -1764,10 +2089,14 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
         else
             hipLaunchKernelGGL((k_extend<B, kStackSlots, false, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
-        if (!BATCH && guide)
-            hipLaunchKernelGGL((k_shade<B, true, false>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
-        else
-            hipLaunchKernelGGL((k_shade<B, false, BATCH>), dim3(ls.grid_stream * 2), dim3(B), 0, st, s, b.cfg, b, d);
+        const dim3 gs(ls.grid_stream * 2);
+        if (!BATCH && guide) {
+            if (cfg.wide) hipLaunchKernelGGL((k_shade<B, true, false, true>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+            else hipLaunchKernelGGL((k_shade<B, true, false, false>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+        } else {
+            if (cfg.wide) hipLaunchKernelGGL((k_shade<B, false, BATCH, true>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+            else hipLaunchKernelGGL((k_shade<B, false, BATCH, false>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+        }
         if (d < cfg.max_depth) {
             hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
             hipLaunchKernelGGL((k_mis<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);

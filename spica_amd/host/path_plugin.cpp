@@ -247,16 +247,56 @@ spk_material Dielectric::describe() const {
     m.int_ior = ior_;
     return m;
 }
-RoughConductor::RoughConductor(RenderParams& p)
-    : eta_(p.getSpectrum("eta")), k_(p.getSpectrum("k")), alpha_(p.getDouble("alpha", 0.1)) {
-    std::string dist = p.getString("distribution", std::string("beckmann"), true);
-    if (dist != "ggx") throw ParamError("RoughConductor: only distribution \"ggx\" is on the GPU path (got \"" + dist + "\")");
+int microfacetFromName(const std::string& name) {
+    if (name == "beckmann") return SPK_MICROFACET_BECKMANN;
+    if (name == "ggx") return SPK_MICROFACET_GGX;
+    throw ParamError("unknown microfacet distribution \"" + name + "\" (beckmann, ggx)");
 }
+RoughConductor::RoughConductor(RenderParams& p)
+    : eta_(p.getSpectrum("eta")), k_(p.getSpectrum("k")), alpha_(p.getDouble("alpha", 0.1)),
+      dist_(microfacetFromName(p.getString("distribution", std::string("beckmann"), true))) {}
 spk_material RoughConductor::describe() const {
     spk_material m = blank(SPK_MAT_ROUGHCONDUCTOR);
     put(m.eta, eta_);
     put(m.k, k_);
     m.alpha = alpha_;
+    m.distribution = dist_;
+    return m;
+}
+Plastic::Plastic(RenderParams& p)
+    : Kd_(p.getSpectrum("diffuseReflectance")), Ks_(p.getSpectrum("specularReflectance")),
+      ior_(p.getDouble("intIOR", 1.5)) {}
+spk_material Plastic::describe() const {
+    spk_material m = blank(SPK_MAT_PLASTIC);
+    put(m.reflectance, Kd_);
+    put(m.specular, Ks_);
+    m.int_ior = ior_;
+    return m;
+}
+RoughDielectric::RoughDielectric(RenderParams& p)
+    : Kr_(p.getSpectrum("specularReflectance")), Kt_(p.getSpectrum("specularTransmittance")),
+      alpha_(p.getDouble("alpha", 0.1)), ior_(p.getDouble("intIOR", 1.3333)),
+      dist_(microfacetFromName(p.getString("distribution", std::string("beckmann"), true))) {}
+spk_material RoughDielectric::describe() const {
+    spk_material m = blank(SPK_MAT_ROUGHDIELECTRIC);
+    put(m.reflectance, Kr_);
+    put(m.transmittance, Kt_);
+    m.alpha = alpha_;
+    m.int_ior = ior_;
+    m.distribution = dist_;
+    return m;
+}
+RoughPlastic::RoughPlastic(RenderParams& p)
+    : Kd_(p.getSpectrum("diffuseReflectance")), Ks_(p.getSpectrum("specularReflectance")),
+      alpha_(p.getDouble("alpha", 0.1)), ior_(p.getDouble("intIOR", 1.5)),
+      dist_(microfacetFromName(p.getString("distribution", std::string("beckmann"), true))) {}
+spk_material RoughPlastic::describe() const {
+    spk_material m = blank(SPK_MAT_ROUGHPLASTIC);
+    put(m.reflectance, Kd_);
+    put(m.specular, Ks_);
+    m.alpha = alpha_;
+    m.int_ior = ior_;
+    m.distribution = dist_;
     return m;
 }
 

@@ -29,11 +29,15 @@ std::shared_ptr<SurfaceMaterial> material_from(const spk_material& m) {
     Spectrum R(m.reflectance[0], m.reflectance[1], m.reflectance[2]);
     Spectrum T(m.transmittance[0], m.transmittance[1], m.transmittance[2]);
     Spectrum eta(m.eta[0], m.eta[1], m.eta[2]), k(m.k[0], m.k[1], m.k[2]);
+    Spectrum S(m.specular[0], m.specular[1], m.specular[2]);
     switch (m.type) {
     case SPK_MAT_DIFFUSE: return std::make_shared<Diffuse>(R);
     case SPK_MAT_CONDUCTOR: return std::make_shared<Conductor>(eta, k);
     case SPK_MAT_DIELECTRIC: return std::make_shared<Dielectric>(R, T, m.int_ior);
-    default: return std::make_shared<RoughConductor>(eta, k, m.alpha);
+    case SPK_MAT_PLASTIC: return std::make_shared<Plastic>(R, S, m.int_ior);
+    case SPK_MAT_ROUGHDIELECTRIC: return std::make_shared<RoughDielectric>(R, T, m.alpha, m.int_ior, m.distribution);
+    case SPK_MAT_ROUGHPLASTIC: return std::make_shared<RoughPlastic>(R, S, m.alpha, m.int_ior, m.distribution);
+    default: return std::make_shared<RoughConductor>(eta, k, m.alpha, m.distribution);
     }
 }
 }  // namespace

@@ -7,7 +7,8 @@
 //   ("directlighting": libspica_directlighting.so -> DirectLightingIntegrator, directlighting.h:28)
 //   integrator->render(camera, scene, params)                               (core/integrator.h:34-36)
 // Scene objects (Triangle, GeometricPrimitive, Diffuse, Conductor, Dielectric,
-// RoughConductor, AreaLight, Envmap, PerspectiveCamera, HDRFilm) take spica's
+// RoughConductor, Plastic, RoughDielectric, RoughPlastic, AreaLight, Envmap,
+// PerspectiveCamera, HDRFilm) take spica's
 // parameter keys and can describe themselves as the flat descriptors of
 // include/spk_types.h, which is what crosses the C ABI into libspk.so.
 #pragma once
@@ -165,15 +166,56 @@ private:
     Spectrum Kr_, Kt_;
     double ior_;
 };
-class RoughConductor : public SurfaceMaterial {  // "eta", "k", "alpha", "distribution" = "ggx"
+// "distribution": "beckmann" (spica's default) or "ggx" (core/microfacet.cc:196-356)
+int microfacetFromName(const std::string& name);
+
+class RoughConductor : public SurfaceMaterial {  // "eta", "k", "alpha", "distribution"
 public:
-    RoughConductor(const Spectrum& eta, const Spectrum& k, double alpha) : eta_(eta), k_(k), alpha_(alpha) {}
+    RoughConductor(const Spectrum& eta, const Spectrum& k, double alpha, int distribution = SPK_MICROFACET_GGX)
+        : eta_(eta), k_(k), alpha_(alpha), dist_(distribution) {}
     explicit RoughConductor(RenderParams& params);
     spk_material describe() const override;
 
 private:
     Spectrum eta_, k_;
     double alpha_;
+    int dist_;
+};
+class Plastic : public SurfaceMaterial {  // "diffuseReflectance", "specularReflectance", "intIOR" (bsdfs/plastic.cc:96-102)
+public:
+    Plastic(const Spectrum& Kd, const Spectrum& Ks, double intIOR) : Kd_(Kd), Ks_(Ks), ior_(intIOR) {}
+    explicit Plastic(RenderParams& params);
+    spk_material describe() const override;
+
+private:
+    Spectrum Kd_, Ks_;
+    double ior_;
+};
+// "specularReflectance", "specularTransmittance", "alpha", "intIOR", "distribution" (roughdielectric.cc:31-40)
+class RoughDielectric : public SurfaceMaterial {
+public:
+    RoughDielectric(const Spectrum& Kr, const Spectrum& Kt, double alpha, double intIOR, int distribution)
+        : Kr_(Kr), Kt_(Kt), alpha_(alpha), ior_(intIOR), dist_(distribution) {}
+    explicit RoughDielectric(RenderParams& params);
+    spk_material describe() const override;
+
+private:
+    Spectrum Kr_, Kt_;
+    double alpha_, ior_;
+    int dist_;
+};
+// "diffuseReflectance", "specularReflectance", "intIOR", "alpha", "distribution" (roughplastic.cc:111-118)
+class RoughPlastic : public SurfaceMaterial {
+public:
+    RoughPlastic(const Spectrum& Kd, const Spectrum& Ks, double alpha, double intIOR, int distribution)
+        : Kd_(Kd), Ks_(Ks), alpha_(alpha), ior_(intIOR), dist_(distribution) {}
+    explicit RoughPlastic(RenderParams& params);
+    spk_material describe() const override;
+
+private:
+    Spectrum Kd_, Ks_;
+    double alpha_, ior_;
+    int dist_;
 };
 
 // ---- shapes, primitives, lights --------------------------------------------------------

@@ -34,9 +34,44 @@ def dielectric(int_ior=1.333, reflectance=(1.0, 1.0, 1.0), transmittance=(1.0, 1
                 transmittance=tuple(map(float, transmittance)))
 
 
-def roughconductor(alpha=0.1, eta=(1.0, 1.0, 1.0), k=(0.0, 0.0, 0.0)):
-    """bsdfs/roughconductor.cc with distribution "ggx": "alpha" (default 0.1), "eta", "k"."""
-    return dict(type=_abi.SPK_MAT_ROUGHCONDUCTOR, alpha=float(alpha), eta=tuple(map(float, eta)), k=tuple(map(float, k)))
+_DISTRIBUTIONS = {"ggx": _abi.SPK_MICROFACET_GGX, "beckmann": _abi.SPK_MICROFACET_BECKMANN}
+
+
+def _distribution(name):
+    if name not in _DISTRIBUTIONS:
+        raise ValueError(f"microfacet distribution {name!r} is not one of {sorted(_DISTRIBUTIONS)}")
+    return _DISTRIBUTIONS[name]
+
+
+def roughconductor(alpha=0.1, eta=(1.0, 1.0, 1.0), k=(0.0, 0.0, 0.0), distribution="ggx"):
+    """bsdfs/roughconductor.cc: "alpha" (default 0.1), "eta", "k", "distribution" ("ggx" here by
+    default; spica's own default, "beckmann", is what its scene files get when they name none)."""
+    return dict(type=_abi.SPK_MAT_ROUGHCONDUCTOR, alpha=float(alpha), eta=tuple(map(float, eta)), k=tuple(map(float, k)),
+                distribution=_distribution(distribution))
+
+
+def plastic(diffuse_reflectance=(0.5, 0.5, 0.5), specular_reflectance=(1.0, 1.0, 1.0), int_ior=1.5):
+    """bsdfs/plastic.cc: "diffuseReflectance", "specularReflectance", "intIOR" (default 1.5)."""
+    return dict(type=_abi.SPK_MAT_PLASTIC, reflectance=tuple(map(float, diffuse_reflectance)),
+                specular=tuple(map(float, specular_reflectance)), int_ior=float(int_ior))
+
+
+def roughdielectric(alpha=0.1, int_ior=1.3333, reflectance=(1.0, 1.0, 1.0), transmittance=(1.0, 1.0, 1.0),
+                    distribution="beckmann"):
+    """bsdfs/roughdielectric.cc: "specularReflectance", "specularTransmittance", "alpha" (0.1),
+    "intIOR" (1.3333), "distribution" ("beckmann")."""
+    return dict(type=_abi.SPK_MAT_ROUGHDIELECTRIC, alpha=float(alpha), int_ior=float(int_ior),
+                reflectance=tuple(map(float, reflectance)), transmittance=tuple(map(float, transmittance)),
+                distribution=_distribution(distribution))
+
+
+def roughplastic(alpha=0.1, diffuse_reflectance=(0.5, 0.5, 0.5), specular_reflectance=(1.0, 1.0, 1.0), int_ior=1.5,
+                 distribution="beckmann"):
+    """bsdfs/roughplastic.cc: "diffuseReflectance", "specularReflectance", "intIOR" (1.5),
+    "alpha" (0.1, at least 0.001), "distribution" ("beckmann")."""
+    return dict(type=_abi.SPK_MAT_ROUGHPLASTIC, alpha=float(alpha), int_ior=float(int_ior),
+                reflectance=tuple(map(float, diffuse_reflectance)), specular=tuple(map(float, specular_reflectance)),
+                distribution=_distribution(distribution))
 
 
 # Conductor optical constants (RGB) used by the procedural scenes: silver for
@@ -48,7 +83,8 @@ GOLD = dict(eta=(0.143, 0.374, 1.442), k=(3.983, 2.385, 1.603))
 def _material_struct(m):
     s = _abi.SpkMaterial()
     s.type = m["type"]
-    for key in ("reflectance", "transmittance", "eta", "k"):
+    s.distribution = m.get("distribution", _abi.SPK_MICROFACET_GGX)
+    for key in ("reflectance", "transmittance", "eta", "k", "specular"):
         v = m.get(key, (0.0, 0.0, 0.0))
         getattr(s, key)[:] = v
     s.int_ior = m.get("int_ior", 1.333)

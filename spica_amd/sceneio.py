@@ -25,7 +25,8 @@ Supported plugins are the ones on the path tracer's path: integrators "path" and
 "directlighting"; sensor
 "perspective"; film "hdrfilm"; samplers "independent" (and "sobol", "halton",
 "ldsampler", which map to the counter-based Sobol stream); bsdfs "diffuse",
-"conductor", "dielectric", "roughconductor" with distribution "ggx"; shapes "ply" and
+"conductor", "dielectric", "roughconductor", "plastic", "roughdielectric" and
+"roughplastic" with distribution "beckmann" (the default) or "ggx"; shapes "ply" and
 "obj"; emitters "area" and "envmap" (RGBE .hdr).  Anything else raises ValueError.
 """
 import math
@@ -36,7 +37,8 @@ import numpy as np
 
 from . import _abi
 from .meshio import load_ply
-from .scene import Camera, Scene, conductor, dielectric, diffuse, look_at, roughconductor
+from .scene import (Camera, Scene, conductor, dielectric, diffuse, look_at, plastic, roughconductor, roughdielectric,
+                    roughplastic)
 
 # ---------------------------------------------------------------- RGBE images
 def read_rgbe(path):
@@ -242,23 +244,45 @@ class XmlScene:
                         self.sampler, self.seed, integrator=kind)
 
 
+def _gray(v):
+    """A texture's gray() (spectrum.cc: 0.2126 r + 0.7152 g + 0.0722 b) of a float or RGB value."""
+    if np.ndim(v) == 0:
+        return float(v)
+    v = np.asarray(v, dtype=np.float64)
+    return float(0.2126 * v[0] + 0.7152 * v[1] + 0.0722 * v[2])
+
+
+def _dist(p):
+    dist = p.take("distribution", "beckmann")
+    if dist not in ("beckmann", "ggx"):
+        raise ValueError(f"microfacet distribution '{dist}' is not supported (beckmann, ggx)")
+    return dist
+
+
 def _material(kind, p):
+    """bsdfs/*.cc (RenderParams constructors): the keys, defaults and gray() reductions spica
+    applies; a texture spica dereferences without a default is required here."""
     if kind == "diffuse":
         return diffuse(p.take("reflectance", required=True))
     if kind == "conductor":
         return conductor(eta=p.take("eta", required=True), k=p.take("k", required=True))
     if kind == "dielectric":
-        ior = p.take("intIOR", 1.333)
-        ior = float(ior[0]) if np.ndim(ior) else float(ior)
-        return dielectric(ior, reflectance=p.take("specularReflectance", (1.0, 1.0, 1.0)),
+        return dielectric(_gray(p.take("intIOR", 1.333)), reflectance=p.take("specularReflectance", (1.0, 1.0, 1.0)),
                           transmittance=p.take("specularTransmittance", (1.0, 1.0, 1.0)))
     if kind == "roughconductor":
-        dist = p.take("distribution", "beckmann")
-        if dist != "ggx":
-            raise ValueError(f"roughconductor distribution '{dist}' is not supported (only ggx)")
-        alpha = p.take("alpha", 0.1)
-        alpha = float(alpha[0]) if np.ndim(alpha) else float(alpha)
-        return roughconductor(alpha=alpha, eta=p.take("eta", required=True), k=p.take("k", required=True))
+        return roughconductor(alpha=_gray(p.take("alpha", 0.1)), eta=p.take("eta", required=True),
+                              k=p.take("k", required=True), distribution=_dist(p))
+    if kind == "plastic":
+        return plastic(p.take("diffuseReflectance", required=True), p.take("specularReflectance", required=True),
+                       _gray(p.take("intIOR", 1.5)))
+    if kind == "roughdielectric":
+        return roughdielectric(alpha=_gray(p.take("alpha", 0.1)), int_ior=_gray(p.take("intIOR", 1.3333)),
+                               reflectance=p.take("specularReflectance", required=True),
+                               transmittance=p.take("specularTransmittance", required=True), distribution=_dist(p))
+    if kind == "roughplastic":
+        return roughplastic(alpha=_gray(p.take("alpha", 0.1)), diffuse_reflectance=p.take("diffuseReflectance", required=True),
+                            specular_reflectance=p.take("specularReflectance", required=True),
+                            int_ior=_gray(p.take("intIOR", 1.5)), distribution=_dist(p))
     raise ValueError(f"bsdf type '{kind}' is not supported")
 
 

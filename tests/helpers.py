@@ -171,6 +171,32 @@ def furnace_room(albedo=0.5, Le=1.0, size=1.0):
     return s, cam
 
 
+def wide_materials_scene():
+    """A 4 x 3 wall of large spheres, one BSDF kind each (plastic, rough dielectric, rough
+    plastic, both microfacet distributions, and the path's original four), over a diffuse
+    floor under two quad lights; the camera sees mostly spheres."""
+    from spica_amd.scene import (GOLD, SILVER, conductor, dielectric, diffuse, plastic, quad, roughconductor,
+                                 roughdielectric, roughplastic, unit_sphere)
+    s = Scene()
+    mats = [diffuse((0.7, 0.3, 0.2)), conductor(**SILVER), dielectric(1.5), roughconductor(0.2, **GOLD),
+            roughconductor(0.3, distribution="beckmann", **GOLD), plastic((0.2, 0.5, 0.8), (1.0, 1.0, 1.0), 1.5),
+            roughdielectric(0.2, 1.5, distribution="beckmann"), roughdielectric(0.1, 1.5, distribution="ggx"),
+            roughplastic(0.15, (0.8, 0.6, 0.2), distribution="beckmann"), roughplastic(0.3, (0.3, 0.8, 0.3), distribution="ggx"),
+            plastic((0.9, 0.9, 0.9), (0.5, 0.5, 0.5), 1.3), roughdielectric(0.4, 1.33, distribution="beckmann")]
+    unit = unit_sphere()
+    for i, m in enumerate(mats):
+        cx, cy = -3.0 + 2.0 * (i % 4), -1.0 + 2.0 * (i // 4)
+        s.add_triangles(unit * 0.95 + np.array([cx, cy, 0.0]), m)
+    s.add_triangles(quad((-8, -2.2, -8), (-8, -2.2, 8), (8, -2.2, 8), (8, -2.2, -8)), diffuse((0.5, 0.5, 0.5)))
+    black = s.add_material(diffuse((0.0, 0.0, 0.0)))
+    for cx in (-3.0, 3.0):
+        y, h = 5.0, 1.0
+        s.add_triangles(quad((cx - h, y, -h - 2), (cx + h, y, -h - 2), (cx + h, y, h - 2), (cx - h, y, h - 2)), black,
+                        radiance=(15.0, 15.0, 15.0))
+    cam = Camera(48, 32, look_at((0.0, 0.5, -9.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0)), 40.0)
+    return s, cam
+
+
 def small_mixed_scene(seed=3, n_spheres=24):
     """Sphere field with every material kind, for whole-path parity at small size."""
     s, _ = configs.sphere_field(seed=seed, n_spheres=n_spheres)

@@ -133,3 +133,29 @@ def test_directlighting_plugin_equals_c_abi_render(gpu, tmp_path):
     gr.close()
     gs.close()
     assert np.array_equal(img, s / (c[..., None] + 1e-12))
+
+
+@pytest.mark.gpu
+def test_plugin_wide_materials_equal_c_abi_render(gpu, tmp_path):
+    """Plastic, rough dielectric, rough plastic and Beckmann materials built through the
+    plugin's spica-style classes give the C ABI's film bit for bit."""
+    from tests.helpers import wide_materials_scene
+
+    s, cam = wide_materials_scene()
+    sf = tmp_path / "s.bin"
+    s.save_bin(sf, cam, 16, 16, 0, 0)
+    out = tmp_path / "img.f64"
+    r = subprocess.run([DEMO, str(sf), str(out), PLUGIN], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    img = np.fromfile(out).reshape(cam.height, cam.width, 3)
+    from spica_amd import configs as cf
+
+    wl = cf.Workload("wide", s, cam, 16, 16, 0, 0)
+    gs = gpu.GpuScene(s)
+    gr = gpu.GpuRender(gs, cam, wl.render_desc())
+    gr.run(16)
+    sm, q, c = gr.film()
+    gr.close()
+    gs.close()
+    assert np.array_equal(img, sm / (c[..., None] + 1e-12))

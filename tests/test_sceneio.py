@@ -160,9 +160,37 @@ def test_unsupported_plugins_raise(tmp_path):
     with pytest.raises(ValueError, match="bdpt"):
         sceneio.load_xml(p)
     p.write_text('<scene><bsdf type="roughconductor" id="r"><rgb name="eta" value="1"/><rgb name="k" value="1"/>'
-                 '</bsdf></scene>')
-    with pytest.raises(ValueError, match="beckmann"):
+                 '<string name="distribution" value="phong"/></bsdf></scene>')
+    with pytest.raises(ValueError, match="phong"):
         sceneio.load_xml(p)
+    p.write_text('<scene><bsdf type="roughplastic" id="r"><rgb name="diffuseReflectance" value="0.5"/></bsdf></scene>')
+    with pytest.raises(ValueError, match="specularReflectance"):
+        sceneio.load_xml(p)
+
+
+def _params(**kv):
+    p = sceneio._Params()
+    for k, v in kv.items():
+        p.add(k, v)
+    return p
+
+
+def test_wide_material_keys_and_defaults():
+    """The plugins' keys and defaults (plastic.cc:96-102, roughdielectric.cc:31-40,
+    roughplastic.cc:111-118, roughconductor.cc:29-36) and gray() of RGB values."""
+    a = sceneio._material("roughconductor", _params(eta=(0.2,) * 3, k=(3.0,) * 3))
+    assert a["distribution"] == _abi.SPK_MICROFACET_BECKMANN and a["alpha"] == 0.1
+    b = sceneio._material("plastic", _params(diffuseReflectance=(0.5, 0.4, 0.3), specularReflectance=(1.0,) * 3))
+    assert b["type"] == _abi.SPK_MAT_PLASTIC and b["int_ior"] == 1.5 and b["specular"] == (1.0,) * 3
+    c = sceneio._material("roughdielectric", _params(specularReflectance=(1.0,) * 3, specularTransmittance=(1.0,) * 3,
+                                                     distribution="ggx", intIOR=(1.2, 1.5, 1.8)))
+    assert c["distribution"] == _abi.SPK_MICROFACET_GGX and c["alpha"] == 0.1
+    assert abs(c["int_ior"] - (0.2126 * 1.2 + 0.7152 * 1.5 + 0.0722 * 1.8)) < 1e-12
+    d = sceneio._material("roughplastic", _params(diffuseReflectance=(0.5,) * 3, specularReflectance=(1.0,) * 3,
+                                                  alpha=0.3))
+    assert d["distribution"] == _abi.SPK_MICROFACET_BECKMANN and d["alpha"] == 0.3 and d["int_ior"] == 1.5
+    assert sceneio._material("roughdielectric", _params(specularReflectance=(1.0,) * 3, specularTransmittance=(1.0,) * 3)
+                             )["int_ior"] == 1.3333
 
 
 def test_envmap_emitter_loads_rgbe(tmp_path):
