@@ -6,10 +6,10 @@
 // different: few, fat, cache-line-sized nodes and short leaves.  This builder
 // bins all three axes (32 bins), stops at up to kMaxLeafTris triangles per leaf
 // when the SAH says a leaf is cheaper, builds large subtrees on parallel threads
-// and then flattens depth-first into 64-byte nodes that hold both child boxes
-// (one node fetch = two box tests).  The closest hit does not depend on the tree
-// (only exact-tie winners do), so the flattened tree is free to differ from
-// spica's.
+// and then flattens depth-first into binary nodes, which collapse_bvh4 turns into the
+// 128-byte 4-wide nodes the traversal reads (one node fetch = four box tests).  The
+// closest hit does not depend on the tree (only exact-tie winners do), so the tree is
+// free to differ from spica's.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -298,6 +298,88 @@ BvhResult build_bvh(const float* verts, int64_t n, const BvhBuildOpts& opts) {
     res.order.resize((size_t)n);
     for (int64_t i = 0; i < n; i++) res.order[(size_t)i] = refs[(size_t)i].id;
     return res;
+}
+
+// ---- 4-wide collapse ------------------------------------------------------------------
+// Each 4-wide node takes the two children of a binary node and keeps opening the interior
+// child of largest surface area into its two children until it holds four (or only
+// leaves): the 4-wide tree visits about half the nodes with the same leaves and the same
+// triangle order.  Nodes are numbered depth first from the root (0).
+int collapse_bvh4(const std::vector<BvhNodeHost>& n2, std::vector<Bvh4NodeHost>& out) {
+    out.clear();
+    if (n2.empty()) return 0;
+    struct Item {
+        float lo[3], hi[3];
+        int32_t link;
+    };
+    auto child = [&](int32_t node, int side) {
+        const BvhNodeHost& h = n2[(size_t)node];
+        Item it;
+        const float* xy = side == 0 ? h.a : h.b;
+        it.lo[0] = xy[0];
+        it.hi[0] = xy[1];
+        it.lo[1] = xy[2];
+        it.hi[1] = xy[3];
+        it.lo[2] = h.c[2 * side];
+        it.hi[2] = h.c[2 * side + 1];
+        it.link = h.d[side];
+        return it;
+    };
+    auto area = [](const Item& it) {
+        const float dx = it.hi[0] - it.lo[0], dy = it.hi[1] - it.lo[1], dz = it.hi[2] - it.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    struct Work {
+        int32_t node2, node4, depth;
+    };
+    std::vector<Work> st;
+    out.push_back(Bvh4NodeHost{});
+    st.push_back({0, 0, 1});
+    int max_depth = 0;
+    while (!st.empty()) {
+        const Work w = st.back();
+        st.pop_back();
+        max_depth = std::max(max_depth, w.depth);
+        Item items[4];
+        int n = 0;
+        items[n++] = child(w.node2, 0);
+        items[n++] = child(w.node2, 1);
+        while (n < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int j = 0; j < n; j++)
+                if (items[j].link >= 0 && area(items[j]) > best_area) {
+                    best = j;
+                    best_area = area(items[j]);
+                }
+            if (best < 0) break;
+            const int32_t opened = items[best].link;
+            items[best] = child(opened, 0);
+            items[n++] = child(opened, 1);
+        }
+        Bvh4NodeHost h;
+        memset(&h, 0, sizeof(h));
+        for (int j = 0; j < 4; j++) {
+            if (j >= n) {
+                h.link[j] = kBvh4Empty;
+                continue;
+            }
+            for (int a = 0; a < 3; a++) {
+                h.lo[a][j] = items[j].lo[a];
+                h.hi[a][j] = items[j].hi[a];
+            }
+            if (items[j].link >= 0) {
+                const int32_t idx = (int32_t)out.size();
+                out.push_back(Bvh4NodeHost{});
+                st.push_back({items[j].link, idx, w.depth + 1});
+                h.link[j] = idx;
+            } else {
+                h.link[j] = items[j].link;
+            }
+        }
+        out[(size_t)w.node4] = h;
+    }
+    return max_depth;
 }
 
 }  // namespace spk

@@ -11,8 +11,9 @@
 //      SAH cost; a subtree of at most max_leaf triangles whose cost as one leaf is no
 //      higher than as a subtree is marked to collapse (the rule of Karras and Aila, "Fast
 //      parallel construction of high-quality BVHs", HPG 2013);
-//   4. emit the 64-byte nodes the traversal reads, collapsed subtrees as leaves over their
-//      contiguous ranges, then renumber the reachable nodes depth first on the host.
+//   4. emit binary nodes (collapsed subtrees as leaves over their contiguous ranges), then
+//      renumber the reachable nodes depth first on the host; the scene then collapses
+//      them into 4-wide nodes like the SAH tree's.
 // Node 0 is the root, as the traversal expects.  The tree answers the same closest hits
 // as the SAH tree (only exact-tie winners may differ); it is cheaper to build and somewhat
 // slower to traverse.

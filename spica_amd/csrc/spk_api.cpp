@@ -388,6 +388,8 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     } else {
         bvh = build_bvh(d->verts, n, bo);
     }
+    std::vector<Bvh4NodeHost> nodes4;
+    const int depth4 = collapse_bvh4(bvh.nodes, nodes4);
     auto t1 = std::chrono::steady_clock::now();
 
     // Leaf-ordered triangle data (FP64 set-up, FP32 storage).
@@ -587,8 +589,7 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     }
 
     // Upload.
-    std::vector<BvhNodeHost>& nodes = bvh.nodes;
-    if ((rc = s->nodes.upload(nodes.data(), nodes.size() * sizeof(BvhNodeHost)))) return rc;
+    if ((rc = s->nodes.upload(nodes4.data(), nodes4.size() * sizeof(Bvh4NodeHost)))) return rc;
     if ((rc = s->tris.upload(tris.data(), tris.size() * sizeof(float4)))) return rc;
     if ((rc = s->tri_ng.upload(tri_ng.data(), tri_ng.size() * sizeof(float4)))) return rc;
     if ((rc = s->tri_ids.upload(tri_ids.data(), tri_ids.size() * sizeof(int2)))) return rc;
@@ -616,7 +617,9 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     }
     sd.env = env;
 
-    s->spill_cap = std::max(8, bvh.max_depth + 2);
+    // a 4-wide visit pushes at most three children, so a ray's stack never holds more
+    // than 3 per level below the root
+    s->spill_cap = std::max(8, 3 * depth4 + 2);
     for (int k = 0; k < 3; k++) {
         s->lo[k] = 3.0e38f;
         s->hi[k] = -3.0e38f;
@@ -631,9 +634,9 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
             s->hi[k] = 1.0f;
         }
     s->stats.num_tris = n;
-    s->stats.num_nodes = (int64_t)nodes.size();
+    s->stats.num_nodes = (int64_t)nodes4.size();
     s->stats.num_leaves = bvh.num_leaves;
-    s->stats.max_depth = bvh.max_depth;
+    s->stats.max_depth = depth4;
     s->stats.num_lights = sd.num_lights;
     s->stats.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     s->stats.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();

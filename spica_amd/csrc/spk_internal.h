@@ -49,6 +49,18 @@ struct BvhResult {
 };
 
 BvhResult build_bvh(const float* verts, int64_t n, const BvhBuildOpts& opts);
+
+// 4-wide node as the traversal reads it: eight float4 = lo.x, lo.y, lo.z, hi.x, hi.y,
+// hi.z of the four children, their links (>= 0 node, < 0 leaf, kBvh4Empty unused), pad.
+static constexpr int32_t kBvh4Empty = 0x7fffffff;
+struct Bvh4NodeHost {
+    float lo[3][4];
+    float hi[3][4];
+    int32_t link[4];
+    int32_t pad[4];
+};
+// Collapses a binary tree (root 0) into 4-wide nodes; returns the 4-wide depth.
+int collapse_bvh4(const std::vector<BvhNodeHost>& nodes2, std::vector<Bvh4NodeHost>& out);
 // Linear BVH on the device (Morton order, radix tree, bottom-up boxes); same node format.
 int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhResult& res);
 

@@ -132,21 +132,53 @@ struct Counters {
     uint32_t nodes, tris;
 };
 
-// Both child boxes of a node against the ray, in FMA form: plane t = p * (1/d) - o * (1/d)
-// (one fused multiply-add per plane), entry clamped at 0 and exit at the current best t.
-// Node layout: a = (L.lo.x, L.hi.x, L.lo.y, L.hi.y), b = same for R, c = (L.lo.z, L.hi.z, R.lo.z, R.hi.z).
-__device__ __forceinline__ void slab2(float4 a, float4 b, float4 c, float3 inv, float3 oinv, float tbest, float& tn0,
-                                      float& tf0, float& tn1, float& tf1) {
-    const float l0x = fmaf(a.x, inv.x, -oinv.x), h0x = fmaf(a.y, inv.x, -oinv.x);
-    const float l0y = fmaf(a.z, inv.y, -oinv.y), h0y = fmaf(a.w, inv.y, -oinv.y);
-    const float l0z = fmaf(c.x, inv.z, -oinv.z), h0z = fmaf(c.y, inv.z, -oinv.z);
-    const float l1x = fmaf(b.x, inv.x, -oinv.x), h1x = fmaf(b.y, inv.x, -oinv.x);
-    const float l1y = fmaf(b.z, inv.y, -oinv.y), h1y = fmaf(b.w, inv.y, -oinv.y);
-    const float l1z = fmaf(c.z, inv.z, -oinv.z), h1z = fmaf(c.w, inv.z, -oinv.z);
-    tn0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), 0.0f));
-    tf0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tbest));
-    tn1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), 0.0f));
-    tf1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tbest));
+
+// One 4-wide node (8 float4: lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of the four children,
+// their links, pad): slab-tests the four boxes against [0, tbest] in FMA form (plane t =
+// p * (1/d) - o * (1/d), one fused multiply-add per plane), continues with the
+// nearest hit child and pushes the other hit children farthest first, so they pop in
+// near-to-far order.  Returns the next node or leaf link (kSent when the stack is empty).
+template <bool COUNT, class Push, class Pop>
+__device__ __forceinline__ int32_t visit4(const float4* __restrict__ nodes, int32_t node, float3 inv, float3 oinv,
+                                          float tbest, Push& push, Pop& pop, Counters& cnt) {
+    const float4* nd = nodes + 8 * node;
+    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5], lk = nd[6];
+    if (COUNT) cnt.nodes++;
+    const float kNone = __int_as_float(0x7f800000);  // +inf: sorts after every hit
+    auto slab = [&](float ax, float ay, float az, float bx, float by, float bz, int32_t link) {
+        const float l_x = fmaf(ax, inv.x, -oinv.x), h_x = fmaf(bx, inv.x, -oinv.x);
+        const float l_y = fmaf(ay, inv.y, -oinv.y), h_y = fmaf(by, inv.y, -oinv.y);
+        const float l_z = fmaf(az, inv.z, -oinv.z), h_z = fmaf(bz, inv.z, -oinv.z);
+        const float tn = fmaxf(fmaxf(fminf(l_x, h_x), fminf(l_y, h_y)), fmaxf(fminf(l_z, h_z), 0.0f));
+        const float tf = fminf(fminf(fmaxf(l_x, h_x), fmaxf(l_y, h_y)), fminf(fmaxf(l_z, h_z), tbest));
+        return (tn <= tf * kRobust && link != kSent) ? tn : kNone;
+    };
+    int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y), c2 = __float_as_int(lk.z), c3 = __float_as_int(lk.w);
+    float k0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, c0);
+    float k1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, c1);
+    float k2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, c2);
+    float k3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, c3);
+    const int n = (k0 != kNone) + (k1 != kNone) + (k2 != kNone) + (k3 != kNone);
+    if (n == 0) return pop();
+    auto cx = [](float& ka, int32_t& ca, float& kb, int32_t& cb) {
+        const bool sw = kb < ka;
+        const float kt = sw ? kb : ka;
+        const int32_t ct = sw ? cb : ca;
+        kb = sw ? ka : kb;
+        cb = sw ? ca : cb;
+        ka = kt;
+        ca = ct;
+    };
+    // ascending by entry distance (misses last): a 5-comparator network
+    cx(k0, c0, k1, c1);
+    cx(k2, c2, k3, c3);
+    cx(k0, c0, k2, c2);
+    cx(k1, c1, k3, c3);
+    cx(k1, c1, k2, c2);
+    if (n > 3) push(c3);
+    if (n > 2) push(c2);
+    if (n > 1) push(c1);
+    return c0;
 }
 
 // Closest hit (ANY = false) or first hit (ANY = true).  Returns the leaf-order
@@ -186,26 +218,7 @@ __device__ __forceinline__ int32_t traverse(const SceneDev& sc, float3 o, float3
     for (;;) {
         // Interior nodes until every lane of the wave holds a leaf (or is done).
         while (node >= 0 && node != kSent) {
-            const float4* nd = nodes + 4 * node;
-            float4 a = nd[0], b = nd[1], c = nd[2];
-            float4 lk = nd[3];
-            if (COUNT) cnt.nodes++;
-            float tn0, tf0, tn1, tf1;
-            slab2(a, b, c, inv, oinv, tbest, tn0, tf0, tn1, tf1);
-            bool h0 = tn0 <= tf0 * kRobust;
-            bool h1 = tn1 <= tf1 * kRobust;
-            int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y);
-            if (h0 && h1) {
-                bool first0 = tn0 <= tn1;
-                node = first0 ? c0 : c1;
-                push(first0 ? c1 : c0);
-            } else if (h0) {
-                node = c0;
-            } else if (h1) {
-                node = c1;
-            } else {
-                node = pop();
-            }
+            node = visit4<COUNT>(nodes, node, inv, oinv, tbest, push, pop, cnt);
             if (node < 0 && leaf == 0) {
                 leaf = node;
                 node = pop();
@@ -355,26 +368,7 @@ __device__ __forceinline__ void traverse_stream(const SceneDev& sc0, const Scene
         const float4* __restrict__ tris = pick_scene<BATCH>(sc, sc0).tris;
         // Interior nodes until every busy lane holds a leaf (or is done).
         while (node >= 0 && node != kSent) {
-            const float4* nd = nodes + 4 * node;
-            float4 a = nd[0], b = nd[1], c = nd[2];
-            float4 lk = nd[3];
-            if (COUNT) cnt.nodes++;
-            float tn0, tf0, tn1, tf1;
-            slab2(a, b, c, inv, oinv, tbest, tn0, tf0, tn1, tf1);
-            bool h0 = tn0 <= tf0 * kRobust;
-            bool h1 = tn1 <= tf1 * kRobust;
-            int32_t c0 = __float_as_int(lk.x), c1 = __float_as_int(lk.y);
-            if (h0 && h1) {
-                bool first0 = tn0 <= tn1;
-                node = first0 ? c0 : c1;
-                push(first0 ? c1 : c0);
-            } else if (h0) {
-                node = c0;
-            } else if (h1) {
-                node = c1;
-            } else {
-                node = pop();
-            }
+            node = visit4<COUNT>(nodes, node, inv, oinv, tbest, push, pop, cnt);
             if (node < 0 && leaf == 0) {
                 leaf = node;
                 node = pop();

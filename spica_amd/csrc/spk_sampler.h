@@ -68,15 +68,19 @@ __host__ __device__ inline uint32_t owen_scramble(uint32_t x, uint32_t seed) {
 }
 
 // Sobol dimension 1 (primitive polynomial x + 1): direction v_0 = 2^31,
-// v_k = v_{k-1} ^ (v_{k-1} >> 1).
+// v_k = v_{k-1} ^ (v_{k-1} >> 1), so v_k has bit 31 - j set iff C(k, j) is odd, i.e.
+// iff j is a bit-subset of k (Lucas).  Bit 31 - j of the point is then the parity of the
+// index bits k that are supersets of j: a superset-parity (zeta) transform over the five
+// bits of k, one shift-xor per bit, followed by a bit reversal.  Equal to the loop
+// x ^= v_k over the set bits k of idx, without its up to 32 data-dependent steps.
 __host__ __device__ inline uint32_t sobol_dim1(uint32_t idx) {
-    uint32_t v = 1u << 31, x = 0;
-    while (idx) {
-        if (idx & 1u) x ^= v;
-        v ^= v >> 1;
-        idx >>= 1;
-    }
-    return x;
+    uint32_t a = idx;
+    a ^= (a >> 1) & 0x55555555u;
+    a ^= (a >> 2) & 0x33333333u;
+    a ^= (a >> 4) & 0x0f0f0f0fu;
+    a ^= (a >> 8) & 0x00ff00ffu;
+    a ^= (a >> 16) & 0x0000ffffu;
+    return bitrev32(a);
 }
 
 __host__ __device__ inline void dim_pair(uint32_t dim, uint32_t& pair, uint32_t& comp) {
