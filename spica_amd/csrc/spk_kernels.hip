@@ -1992,7 +1992,10 @@ __global__ __launch_bounds__(256) void k_reduce_sum(const double* partial, int n
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-constexpr int kStackSlots = 32;
+// LDS stack slots per lane.  16 (4 KiB per 64-lane block) lets the traversal kernels run
+// 7-8 waves per SIMD; a 4-wide tree rarely needs more (entries beyond spill to memory).
+// 32 slots held them at 5 waves: C1 2963 -> 3569 Mrays/s going from 32 to 16.
+constexpr int kStackSlots = 16;
 
 __global__ void k_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
                           uint32_t* out) {
