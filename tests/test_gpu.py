@@ -177,6 +177,32 @@ def test_lbvh_and_sah_agree(gpu):
     assert diff <= n * 1e-4
 
 
+def test_deep_stack_spill_parity(gpu, oracle):
+    """Forty concentric spheres seen from their centre: every level's child boxes contain
+    the origin, so a ray's traversal stack outgrows its 16 LDS slots and spills to memory;
+    closest and any hits still agree with the checker."""
+    import torch
+    from spica_amd.scene import unit_sphere
+
+    s = Scene()
+    unit = unit_sphere()
+    for r in range(1, 41):
+        s.add_triangles(unit * float(r), diffuse((0.5, 0.5, 0.5)))
+    rng = np.random.default_rng(9)
+    n = 200000
+    org = (0.01 * rng.normal(size=(n, 3))).astype(np.float32)
+    dirv = rng.normal(size=(n, 3)).astype(np.float32)
+    pg, tg = _trace_parity(gpu, oracle, s, org, dirv, 1e-3)
+    assert (pg >= 0).all()
+    gs = gpu.GpuScene(s)
+    o, d = torch.from_numpy(org).cuda(), torch.from_numpy(dirv).cuda()
+    occ, _ = gs.trace(o, d, tmax=torch.from_numpy(tg).cuda() * 0.999, any_hit=True)
+    assert float((occ == 1).float().mean()) < 1e-3
+    occ2, _ = gs.trace(o, d, tmax=torch.full((n,), 50.0, device="cuda"), any_hit=True)
+    assert bool((occ2 == 1).all())
+    gs.close()
+
+
 def test_any_hit_matches_closest(gpu):
     import torch
 
