@@ -265,9 +265,13 @@ def main():
         log("bench: C2 batch: generating 64 scenes")
         t0 = time.perf_counter()
         wb = configs.c2_batch()
+        gen_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
         scenes = [render.GpuScene(sc) for sc, _, _, _ in wb.items]
+        scene_s = time.perf_counter() - t0
         tris = sum(g.stats()["num_tris"] for g in scenes)
-        setup_s = time.perf_counter() - t0
+        bvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3
+        setup_s = gen_s + scene_s
         items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
         b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
                          block_size=args.block, graph=1)
@@ -285,7 +289,29 @@ def main():
                      images=len(items), triangles=int(tris), paths=int(info["paths"]), passes=int(info["passes"]),
                      seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
                      mpaths_per_s=round(info["paths"] / bdt / 1e6, 3), setup_s=round(setup_s, 1),
+                     generate_s=round(gen_s, 1), scenes_s=round(scene_s, 1), bvh_build_s=round(bvh_s, 1),
                      queue=int(info["queue"]), block=args.block)
+        b.close()
+        for g in scenes:
+            g.close()
+        # the same batch over trees built on the device: the time to the first pixel
+        t0 = time.perf_counter()
+        scenes = [render.GpuScene(sc, builder="lbvh") for sc, _, _, _ in wb.items]
+        lscene_s = time.perf_counter() - t0
+        lbvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3
+        items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
+        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
+                         block_size=args.block, graph=1)
+        st0 = b.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run(sync=True)
+        torch.cuda.synchronize()
+        ldt = time.perf_counter() - t0
+        st1 = b.stats()
+        lrays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+        batch["lbvh"] = dict(scenes_s=round(lscene_s, 1), bvh_build_s=round(lbvh_s, 1), seconds=round(ldt, 3),
+                             mrays_per_s=round(lrays / ldt / 1e6, 3))
         log(f"bench: batch {batch}")
         b.close()
         for g in scenes:
