@@ -1807,13 +1807,19 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             }
             if (ok) {
                 const float3 f = bsdf_f<false>(b, wo, wi, BX_ALL);
-                const float pdf = (1.0f - net.frac) * bsdf_pdf<false>(b, wo, wi, BX_ALL) +
-                                  net.frac * __expf(mix_log_pdf(mix, wi.x, wi.y, wi.z));
+                const float pb = bsdf_pdf<false>(b, wo, wi, BX_ALL);
+                const float pdf = (1.0f - net.frac) * pb + net.frac * __expf(mix_log_pdf(mix, wi.x, wi.y, wi.z));
                 if (!black(f) && pdf > 0.0f) {
-                    float3 nb = cmul(beta, mul(f, fabsf(dot(wi, b.n)) / pdf));
+                    const float cw = fabsf(dot(wi, b.n));
+                    float3 nb = cmul(beta, mul(f, cw / pdf));
                     bool cont = true;
                     if (depth > rr_depth) {
-                        float q = fminf(0.95f, gray(nb));
+                        // Survival from the throughput BSDF sampling would give this direction,
+                        // beta f cos / p_bsdf: the mixture's pdf is highest where the network expects
+                        // the most light, so beta f cos / p_mix is lowest there, and a roulette on it
+                        // would end the very paths the guide sends towards the light.  Any survival
+                        // probability in (0, 1] with the 1/q reweighting keeps the estimate unbiased.
+                        float q = fminf(0.95f, gray(pb > 0.0f ? cmul(beta, mul(f, cw / pb)) : nb));
                         if (sample_1d(kind, seed, pixel, sample, dim_base + 7) > q) cont = false;
                         else nb = mul(nb, 1.0f / q);
                     }
