@@ -95,6 +95,48 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
     return base + prefix;
 }
 
+// ---- block-level queue append -----------------------------------------------------------
+// One counter atomic per block and queue instead of one per wave.  The queue counters are
+// the hottest addresses of a pass and the atomics on one address are performed one after
+// another, so at a wave per atomic they took about 60% of the shading kernel's time on a
+// full first bounce.  Every thread of the block calls this together (block-uniform loops).
+// lds holds 2 * NQ * (NW + 1) words, used by alternate calls (parity), so a wave that has
+// moved on to the next call never overwrites totals a slower wave still reads: it cannot
+// pass that next call's first barrier before every wave has finished reading.
+template <int NW, int NQ>
+__device__ __forceinline__ void block_append(const bool (&pred)[NQ], uint32_t* const (&counter)[NQ],
+                                             uint32_t (&slot)[NQ], uint32_t* lds, uint32_t parity) {
+    constexpr int R = NW + 1;  // per queue: the wave totals, then the block's base
+    uint32_t* buf = lds + parity * (NQ * R);
+    const int w = (int)(threadIdx.x >> 6);
+    uint32_t prefix[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const unsigned long long m = __ballot(pred[q] ? 1 : 0);
+        prefix[q] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (__lane_id() == 0) buf[q * R + w] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)NQ) {
+        const int q = (int)threadIdx.x;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int v = 0; v < NW; v++) sum += buf[q * R + v];
+        uint32_t* c = counter[0];
+#pragma unroll
+        for (int k = 1; k < NQ; k++)
+            if (q == k) c = counter[k];
+        buf[q * R + NW] = sum != 0 ? atomicAdd(c, sum) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        uint32_t b = buf[q * R + NW];
+        for (int v = 0; v < w; v++) b += buf[q * R + v];
+        slot[q] = b + prefix[q];
+    }
+}
+
 // ---- ray/triangle (triangle.cc:98-178 in float) -----------------------------------------
 __device__ __forceinline__ bool tri_hit(const float4* __restrict__ tris, uint32_t k, float3 o, float3 d, float tmax,
                                         float& t, float& u, float& v) {
@@ -1412,6 +1454,10 @@ __device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
 }
 
 // ---- shade ---------------------------------------------------------------------------------------
+// Shading runs in blocks of its own size, independent of the traversal kernels' block:
+// the queue appends are aggregated per block (block_append).
+constexpr int kShadeBlock = 256;
+
 template <int BLOCK, bool GUIDED, bool BATCH, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                  int depth) {
@@ -1427,6 +1473,8 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
     const uint32_t sample_base = ps->sample_base;
     const int cur = depth & 1, nxt = cur ^ 1;
     const uint32_t dim_base = direct ? 4u + 10u * (uint32_t)depth : 4u + 8u * (uint32_t)depth;
+    __shared__ uint32_t app_lds[2 * 4 * (BLOCK / 64 + 1)];
+    uint32_t parity = 0;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
         const uint32_t i = base + threadIdx.x;
         const bool active = i < count;
@@ -1606,11 +1654,25 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
                 rb.path_L[path] = make_float4(Lv.x + Ladd.x, Lv.y + Ladd.y, Lv.z + Ladd.z, Lv.w);
             }
         }
-        const uint32_t sa = wave_append(pushA, &ps->countA[depth + 1]);
-        const uint32_t ss = wave_append(pushS, &ps->countS[depth]);
-        const uint32_t sm = wave_append(pushM, &ps->countM[depth]);
-        if (GUIDED) {
-            const uint32_t sg = wave_append(pushG, &ps->countG[depth]);
+        constexpr int NQ = GUIDED ? 4 : 3;
+        bool pr[NQ];
+        uint32_t* ctr[NQ];
+        uint32_t sl[NQ];
+        pr[0] = pushA;
+        ctr[0] = &ps->countA[depth + 1];
+        pr[1] = pushS;
+        ctr[1] = &ps->countS[depth];
+        pr[2] = pushM;
+        ctr[2] = &ps->countM[depth];
+        if constexpr (GUIDED) {
+            pr[NQ - 1] = pushG;
+            ctr[NQ - 1] = &ps->countG[depth];
+        }
+        block_append<BLOCK / 64, NQ>(pr, ctr, sl, app_lds, parity);
+        parity ^= 1u;
+        const uint32_t sa = sl[0], ss = sl[1], sm = sl[2];
+        if constexpr (GUIDED) {
+            const uint32_t sg = sl[NQ - 1];
             if (pushG) {
                 rb.qg[0][sg] = gq0;
                 rb.qg[1][sg] = gq1;
@@ -2092,13 +2154,14 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
         else
             hipLaunchKernelGGL((k_extend<B, kStackSlots, false, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
-        const dim3 gs(ls.grid_stream * 2);
+        constexpr int SB = kShadeBlock;
+        const dim3 gs(ls.grid_stream * 2 * 64 / SB), bs(SB);
         if (!BATCH && guide) {
-            if (cfg.wide) hipLaunchKernelGGL((k_shade<B, true, false, true>), gs, dim3(B), 0, st, s, b.cfg, b, d);
-            else hipLaunchKernelGGL((k_shade<B, true, false, false>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, true, false, true>), gs, bs, 0, st, s, b.cfg, b, d);
+            else hipLaunchKernelGGL((k_shade<SB, true, false, false>), gs, bs, 0, st, s, b.cfg, b, d);
         } else {
-            if (cfg.wide) hipLaunchKernelGGL((k_shade<B, false, BATCH, true>), gs, dim3(B), 0, st, s, b.cfg, b, d);
-            else hipLaunchKernelGGL((k_shade<B, false, BATCH, false>), gs, dim3(B), 0, st, s, b.cfg, b, d);
+            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, false, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, d);
+            else hipLaunchKernelGGL((k_shade<SB, false, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, d);
         }
         if (d < cfg.max_depth) {
             hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
