@@ -83,18 +83,6 @@ __device__ __forceinline__ void coordinate_system(float3 w, float3& u, float3& v
     v = normalize(cross(w, u));
 }
 
-// ---- wave-level queue append ----------------------------------------------------------
-// All 64 lanes of the wave must call this together (the loops below are uniform).
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter) {
-    unsigned long long m = __ballot(pred ? 1 : 0);
-    uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    uint32_t total = (uint32_t)__popcll(m);
-    uint32_t base = 0;
-    if (__lane_id() == 0 && total != 0) base = atomicAdd(counter, total);
-    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-    return base + prefix;
-}
-
 // ---- block-level queue append -----------------------------------------------------------
 // One counter atomic per block and queue instead of one per wave.  The queue counters are
 // the hottest addresses of a pass and the atomics on one address are performed one after
@@ -1805,8 +1793,11 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
     const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
     const int lane = __lane_id(), wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     float* ob = obuf[wv];
-    const uint32_t waves = gridDim.x * (kGuideBlock / 64);
-    for (uint32_t base = (blockIdx.x * (kGuideBlock / 64) + wv) * 64u; base < count; base += waves * 64u) {
+    __shared__ uint32_t app_lds[2 * 2 * (kGuideBlock / 64 + 1)];
+    uint32_t parity = 0;
+    // block-uniform trip count: the appends below synchronise the block
+    for (uint32_t bb = blockIdx.x * kGuideBlock; bb < count; bb += gridDim.x * kGuideBlock) {
+        const uint32_t base = bb + (uint32_t)wv * 64u;
         const uint32_t i = base + lane;
         const bool active = i < count;
         float4 G0 = make_float4(0, 0, 0, 0), G1 = make_float4(0, 0, 1, 0), G3 = make_float4(0, 0, 1, 0);
@@ -1900,14 +1891,19 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                 }
             }
         }
-        const uint32_t sa = wave_append(pushA, &ps->countA[depth + 1]);
+        const bool pr[2] = {pushA, pushR};
+        uint32_t* const ctr[2] = {&ps->countA[depth + 1], net.rec_count};
+        uint32_t sl[2];
+        block_append<kGuideBlock / 64, 2>(pr, ctr, sl, app_lds, parity);
+        parity ^= 1u;
+        const uint32_t sa = sl[0];
         if (pushA) {
             rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
             rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
             rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
         }
         if (net.record) {
-            const uint32_t sr = wave_append(pushR, net.rec_count);
+            const uint32_t sr = sl[1];
             if (pushR && sr < net.rec_cap) {
                 const float4 L = rb.path_L[path];
                 net.rec[0][sr] = make_float4(x[0], x[1], x[2], x[3]);
