@@ -765,18 +765,6 @@ __device__ __forceinline__ float ggx_lambda(float alpha, float3 w) {
     float a = al * at;
     return (-1.0f + sqrtf(1.0f + a * a)) / 2.0f;
 }
-__device__ __forceinline__ float ggx_G1(float alpha, float3 w, float3 wh) {
-    if (w.z * dot(w, wh) <= 0.0f) return 0.0f;
-    return 1.0f / (1.0f + ggx_lambda(alpha, w));
-}
-__device__ __forceinline__ float ggx_G(float alpha, float3 wo, float3 wi, float3 wh) {
-    if (wi.z * dot(wi, wh) <= 0.0f) return 0.0f;
-    if (wo.z * dot(wo, wh) <= 0.0f) return 0.0f;
-    return 1.0f / (1.0f + ggx_lambda(alpha, wo) + ggx_lambda(alpha, wi));
-}
-__device__ __forceinline__ float ggx_pdf(float alpha, float3 wo, float3 wh) {
-    return ggx_D(alpha, wh) * ggx_G1(alpha, wo, wh) * fabsf(dot(wo, wh)) / fabsf(wo.z);
-}
 // microfacet.cc:15-56: visible-slope sampling, rational fit for the y slope.
 __device__ __forceinline__ void ggx_sample11(float cosT, float u0, float u1, float& sx, float& sy) {
     if (cosT > 0.9999f) {
@@ -1801,10 +1789,14 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         const uint32_t i = base + lane;
         const bool active = i < count;
         float4 G0 = make_float4(0, 0, 0, 0), G1 = make_float4(0, 0, 1, 0), G3 = make_float4(0, 0, 1, 0);
+        float4 G2 = make_float4(0, 0, 0, 0), NG = make_float4(0, 0, 1, 0);
         if (active) {
             G0 = rb.qg[0][i];
             G1 = rb.qg[1][i];
             G3 = rb.qg[3][i];
+            // issued before the network so that their latency overlaps the MFMA chain
+            G2 = rb.qg[2][i];
+            NG = sc.tri_ng[__float_as_int(G3.w)];
         }
         float x[7];
         guide_inputs(net, xyz(G0), xyz(G3), xyz(G1), x);
@@ -1833,14 +1825,11 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             const float* o = ob + lane * kGuideObStride;
             GuideMix mix;
             mix_from_outputs([&](int u) { return o[u]; }, mix);
-            const float4 G2 = rb.qg[2][i];
             path = __float_as_uint(G0.w);
             pixel = __float_as_uint(G2.w);
             beta = xyz(G2);
             const uint32_t flags = __float_as_uint(G1.w);
             const uint32_t sample = sample_base + (flags >> 7);
-            const int32_t k = __float_as_int(G3.w);
-            const float4 NG = sc.tri_ng[k];
             const uint32_t mbits = __float_as_uint(NG.w);
             const float3 ng = xyz(NG), p = xyz(G0), wo = xyz(G1);
             Bsdf b;
