@@ -100,6 +100,17 @@ def load_reference(name):
     return d["pixels"], d["mean"], int(d["spp"])
 
 
+def load_batch_reference():
+    """65536-spp float64 CPU references of some batch images (tests/golden/make_reference.py c2)."""
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", "c2_ref.npz")
+    if not os.path.exists(path):
+        return None
+    d = np.load(path)
+    return dict(spp=int(d["spp"]), images={int(i): (d[f"pixels_{i}"], d[f"mean_{i}"]) for i in d["images"]})
+
+
 def time_to_relmse(gs, w, ref, guide_cfg, target, max_spp, queue, block, chunk=32):
     """Wall time (render calls only, synchronised) until relMSE against the reference
     falls to `target`, checked every `chunk` samples per pixel on the GPU in float64."""
@@ -291,6 +302,18 @@ def main():
                      mpaths_per_s=round(info["paths"] / bdt / 1e6, 3), setup_s=round(setup_s, 1),
                      generate_s=round(gen_s, 1), scenes_s=round(scene_s, 1), bvh_build_s=round(bvh_s, 1),
                      queue=int(info["queue"]), block=args.block)
+        c2_ref = load_batch_reference()
+        if c2_ref is not None:
+            import numpy as np
+
+            rel = {}
+            for i, (pix, refm) in c2_ref["images"].items():
+                s, _, c = b.film(i)
+                m = (s / np.maximum(c, 1)[..., None]).reshape(-1, 3)[pix]
+                rel[str(i)] = dict(spp=int(wb.items[i][2]),
+                                   relmse=float(f"{float(((m - refm) ** 2 / (refm ** 2 + 1e-2)).mean()):.4g}"))
+            batch["relmse"] = dict(images=rel, eps=1e-2,
+                                   reference=f"{c2_ref['spp']}-spp float64 CPU render, stratified pixels per image")
         b.close()
         for g in scenes:
             g.close()

@@ -5,6 +5,7 @@ estimator, independent sampler with a seed no GPU run uses).
     python tests/golden/make_reference.py c3 [nx]        # 1280x720, 160x90 stratified pixels, 65536 spp
     python tests/golden/make_reference.py c1             # 1920x1080, 160x90 stratified pixels, 65536 spp
     python tests/golden/make_reference.py c4             # view 0 of the heavy scene, 80x45 stratified pixels, 65536 spp
+    python tests/golden/make_reference.py c2 [i ...]     # batch images i (default 0 1 2), 40 x ~40h/w pixels, 65536 spp
 
 Writes tests/golden/<name>_ref.npz with the film indices, per-pixel mean and the
 standard error of the mean.  CPU-only; run in the build container.
@@ -31,7 +32,7 @@ def subset_pixels(w, h, nx, ny):
     return np.array([y * w + x for y in ys for x in xs], dtype=np.int32)
 
 
-def render(name, wl, subset, spp, chunk, threads):
+def render(name, wl, subset, spp, chunk, threads, write=True):
     sc = Oracle().scene(wl.scene)
     npix = len(subset) if subset is not None else wl.camera.width * wl.camera.height
     state = (np.zeros((npix, 3)), np.zeros((npix, 3)), np.zeros(npix, dtype=np.int64))
@@ -43,10 +44,30 @@ def render(name, wl, subset, spp, chunk, threads):
     s, q, c = state
     mean = s / c[:, None]
     var = np.maximum(q / c[:, None] - mean ** 2, 0.0) * c[:, None] / (c[:, None] - 1)
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"{name}_ref.npz")
-    np.savez_compressed(out, pixels=np.arange(npix) if subset is None else subset, mean=mean,
-                        sem=np.sqrt(var / c[:, None]).astype(np.float32), spp=spp, seed=SEED)
-    print("wrote", out)
+    res = dict(pixels=np.arange(npix) if subset is None else subset, mean=mean,
+               sem=np.sqrt(var / c[:, None]).astype(np.float32), spp=spp, seed=SEED)
+    if write:
+        out = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"{name}_ref.npz")
+        np.savez_compressed(out, **res)
+        print("wrote", out)
+    return res
+
+
+def render_batch(images, threads):
+    """References for some images of the batch workload, one npz holding all of them."""
+    wb = configs.c2_batch()
+    out = {"images": np.array(images, dtype=np.int32), "spp": 65536, "seed": SEED}
+    for i in images:
+        sc, cam, _, _ = wb.items[i]
+        wl = configs.Workload(f"C2 image {i}", sc, cam, 65536, wb.max_depth, 0, SEED)
+        nx = 40
+        ny = max(8, int(round(nx * cam.height / cam.width)))
+        r = render(f"c2[{i}]", wl, subset_pixels(cam.width, cam.height, nx, ny), 65536, 4096, threads, write=False)
+        for k in ("pixels", "mean", "sem"):
+            out[f"{k}_{i}"] = r[k]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c2_ref.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path)
 
 
 if __name__ == "__main__":
@@ -62,6 +83,8 @@ if __name__ == "__main__":
     elif which == "c1":
         wl = configs.c1_spheres()
         render("c1", wl, subset_pixels(wl.camera.width, wl.camera.height, 160, 90), 65536, 1024, threads)
+    elif which == "c2":
+        render_batch([int(a) for a in sys.argv[2:]] or [0, 1, 2], threads)
     elif which == "c4":
         wl = configs.c4_heavy()[0]
         render("c4", wl, subset_pixels(wl.camera.width, wl.camera.height, 80, 45), 65536, 1024, threads)
