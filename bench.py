@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = "time to relMSE 1e-3 (guided vs plain PT, 1 GPU); Mrays/s + roofline frac"
 HBM_PEAK_GBS = 8000.0
 B_RAY_IO = 32 + 16          # ray in (o, d, tmax, id) + hit out (t, u, v, prim)
-B_NODE = 128                # one 4-wide BVH node: four child boxes + links
+B_NODE = 64                 # one 4-wide BVH node: four 8-bit-quantised child boxes against the node's box + links
 B_TRI = 48                  # one triangle: v0 | id, e1, e2
 
 

@@ -6,8 +6,9 @@
 // different: few, fat, cache-line-sized nodes and short leaves.  This builder
 // bins all three axes (32 bins), stops at up to kMaxLeafTris triangles per leaf
 // when the SAH says a leaf is cheaper, builds large subtrees on parallel threads
-// and then flattens depth-first into binary nodes, which collapse_bvh4 turns into the
-// 128-byte 4-wide nodes the traversal reads (one node fetch = four box tests).  The
+// and then flattens depth-first into binary nodes, which collapse_bvh4 turns into 4-wide
+// nodes and quantize_bvh4 into the 64-byte nodes the traversal reads (one node fetch =
+// four box tests).  The
 // closest hit does not depend on the tree (only exact-tie winners do), so the tree is
 // free to differ from spica's.
 #include <algorithm>
@@ -380,6 +381,55 @@ int collapse_bvh4(const std::vector<BvhNodeHost>& n2, std::vector<Bvh4NodeHost>&
         out[(size_t)w.node4] = h;
     }
     return max_depth;
+}
+
+// ---- 8-bit child boxes ----------------------------------------------------------------
+// Per axis the node's box [lo, hi] (the union of its children) is split into 255 steps:
+// scale = (hi - lo) / 255, raised until code 255 reaches hi.  A child's lower plane takes the
+// largest code whose decoding fmaf(q, scale, lo) is <= the plane, its upper plane the
+// smallest code whose decoding is >= it.  Device and host fmaf are both correctly rounded,
+// so the traversal decodes exactly the boxes checked here.  A flat axis has scale 0 and
+// decodes every code to the plane itself.
+void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHost>& out) {
+    out.resize(in.size());
+    for (size_t i = 0; i < in.size(); i++) {
+        const Bvh4NodeHost& h = in[i];
+        Bvh4QNodeHost& q = out[i];
+        memset(&q, 0, sizeof(q));
+        for (int c = 0; c < 4; c++) q.link[c] = h.link[c];
+        for (int a = 0; a < 3; a++) {
+            float lo = FLT_MAX, hi = -FLT_MAX;
+            for (int c = 0; c < 4; c++)
+                if (h.link[c] != kBvh4Empty) {
+                    lo = std::min(lo, h.lo[a][c]);
+                    hi = std::max(hi, h.hi[a][c]);
+                }
+            if (lo > hi) lo = hi = 0.0f;  // no children (never for a built tree)
+            float sc = (hi - lo) / 255.0f;
+            while (std::fmaf(255.0f, sc, lo) < hi) sc = std::nextafter(sc, FLT_MAX);
+            q.origin[a] = lo;
+            q.scale[a] = sc;
+            uint32_t wlo = 0, whi = 0;
+            for (int c = 0; c < 4; c++) {
+                int ql = 0, qh = 0;
+                if (h.link[c] != kBvh4Empty) {
+                    const float cl = h.lo[a][c], ch = h.hi[a][c];
+                    if (sc > 0.0f) {
+                        ql = std::min(255, std::max(0, (int)std::floor((cl - lo) / sc)));
+                        qh = std::min(255, std::max(0, (int)std::ceil((ch - lo) / sc)));
+                        while (ql > 0 && std::fmaf((float)ql, sc, lo) > cl) ql--;
+                        while (ql < 255 && std::fmaf((float)(ql + 1), sc, lo) <= cl) ql++;
+                        while (qh < 255 && std::fmaf((float)qh, sc, lo) < ch) qh++;
+                        while (qh > 0 && std::fmaf((float)(qh - 1), sc, lo) >= ch) qh--;
+                    }
+                }
+                wlo |= (uint32_t)ql << (8 * c);
+                whi |= (uint32_t)qh << (8 * c);
+            }
+            q.qlo[a] = wlo;
+            q.qhi[a] = whi;
+        }
+    }
 }
 
 }  // namespace spk

@@ -388,8 +388,13 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     } else {
         bvh = build_bvh(d->verts, n, bo);
     }
-    std::vector<Bvh4NodeHost> nodes4;
-    const int depth4 = collapse_bvh4(bvh.nodes, nodes4);
+    std::vector<Bvh4QNodeHost> nodes4;
+    int depth4 = 0;
+    {
+        std::vector<Bvh4NodeHost> wide;
+        depth4 = collapse_bvh4(bvh.nodes, wide);
+        quantize_bvh4(wide, nodes4);
+    }
     auto t1 = std::chrono::steady_clock::now();
 
     // Leaf-ordered triangle data (FP64 set-up, FP32 storage).
@@ -589,7 +594,7 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     }
 
     // Upload.
-    if ((rc = s->nodes.upload(nodes4.data(), nodes4.size() * sizeof(Bvh4NodeHost)))) return rc;
+    if ((rc = s->nodes.upload(nodes4.data(), nodes4.size() * sizeof(Bvh4QNodeHost)))) return rc;
     if ((rc = s->tris.upload(tris.data(), tris.size() * sizeof(float4)))) return rc;
     if ((rc = s->tri_ng.upload(tri_ng.data(), tri_ng.size() * sizeof(float4)))) return rc;
     if ((rc = s->tri_ids.upload(tri_ids.data(), tri_ids.size() * sizeof(int2)))) return rc;

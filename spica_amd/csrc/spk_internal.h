@@ -61,6 +61,21 @@ struct Bvh4NodeHost {
 };
 // Collapses a binary tree (root 0) into 4-wide nodes; returns the 4-wide depth.
 int collapse_bvh4(const std::vector<BvhNodeHost>& nodes2, std::vector<Bvh4NodeHost>& out);
+
+// The 64-byte node the traversal reads: the four child boxes quantised to 8 bits per plane
+// against the node's own box.  Words: origin x y z, scale x | scale y z, lo x, lo y | lo z,
+// hi x, hi y, hi z | links; byte c of a lo/hi word is child c's code q, decoded as
+// fmaf(q, scale, origin).  Codes are rounded outwards and checked against that exact
+// decoding, so every decoded box contains the child's box.
+struct Bvh4QNodeHost {
+    float origin[3];
+    float scale[3];
+    uint32_t qlo[3];
+    uint32_t qhi[3];
+    int32_t link[4];
+};
+static_assert(sizeof(Bvh4QNodeHost) == 64, "quantised 4-wide node is 64 bytes");
+void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHost>& out);
 // Linear BVH on the device (Morton order, radix tree, bottom-up boxes); same node format.
 int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhResult& res);
 

@@ -171,9 +171,19 @@ struct Counters {
 template <bool COUNT, class Push, class Pop>
 __device__ __forceinline__ int32_t visit4(const float4* __restrict__ nodes, int32_t node, float3 inv, float3 oinv,
                                           float tbest, Push& push, Pop& pop, Counters& cnt) {
-    const float4* nd = nodes + 8 * node;
-    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5], lk = nd[6];
+    // 64-byte node: origin, scale and 8-bit child planes (decoded as fmaf(q, scale, origin),
+    // boxes rounded outwards at build time), then the four links
+    const float4* nd = nodes + 4 * node;
+    const float4 A = nd[0], B = nd[1], C = nd[2], lk = nd[3];
     if (COUNT) cnt.nodes++;
+    const uint32_t qlx = __float_as_uint(B.z), qly = __float_as_uint(B.w), qlz = __float_as_uint(C.x);
+    const uint32_t qhx = __float_as_uint(C.y), qhy = __float_as_uint(C.z), qhz = __float_as_uint(C.w);
+    auto dq = [](uint32_t w, int c, float sc, float o) { return fmaf((float)((w >> (8 * c)) & 255u), sc, o); };
+    auto plane4 = [&](uint32_t w, float sc, float o) {
+        return make_float4(dq(w, 0, sc, o), dq(w, 1, sc, o), dq(w, 2, sc, o), dq(w, 3, sc, o));
+    };
+    const float4 lx = plane4(qlx, A.w, A.x), ly = plane4(qly, B.x, A.y), lz = plane4(qlz, B.y, A.z);
+    const float4 hx = plane4(qhx, A.w, A.x), hy = plane4(qhy, B.x, A.y), hz = plane4(qhz, B.y, A.z);
     const float kNone = __int_as_float(0x7f800000);  // +inf: sorts after every hit
     auto slab = [&](float ax, float ay, float az, float bx, float by, float bz, int32_t link) {
         const float l_x = fmaf(ax, inv.x, -oinv.x), h_x = fmaf(bx, inv.x, -oinv.x);
