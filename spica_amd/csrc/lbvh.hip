@@ -1,8 +1,9 @@
 // lbvh.hip -- linear BVH built on the GPU (Karras, "Maximizing parallelism in the
 // construction of BVHs, octrees, and k-d trees", HPG 2012).
 //
-//   1. 30-bit Morton code of each triangle centroid in the scene box, sorted with its
-//      index (hipCUB radix sort);
+//   1. 63-bit Morton code (21 bits per axis) of each triangle centroid in the scene box,
+//      sorted with its index (hipCUB radix sort); 30-bit codes gave many equal keys on
+//      scenes of small, dense meshes, and equal keys split by index, not by space;
 //   2. one thread per internal node finds the node's key range and split from the
 //      common-prefix lengths of neighbouring codes (ties broken by index), giving a
 //      binary radix tree whose node i covers a contiguous range of sorted triangles;
@@ -30,39 +31,42 @@ namespace spk {
 
 namespace {
 
-__device__ __forceinline__ uint32_t expand10(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
+// Spreads the low 21 bits of v to every third bit of a 64-bit word.
+__device__ __forceinline__ uint64_t expand21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
 }
 
-__global__ void k_morton(const float* __restrict__ v, uint32_t n, float3 lo, float3 scale, uint32_t* codes,
+__global__ void k_morton(const float* __restrict__ v, uint32_t n, float3 lo, float3 scale, uint64_t* codes,
                          uint32_t* idx) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float* t = v + 9 * (size_t)i;
         const float cx = (t[0] + t[3] + t[6]) * (1.0f / 3.0f);
         const float cy = (t[1] + t[4] + t[7]) * (1.0f / 3.0f);
         const float cz = (t[2] + t[5] + t[8]) * (1.0f / 3.0f);
-        const uint32_t x = (uint32_t)fminf(fmaxf((cx - lo.x) * scale.x, 0.0f), 1023.0f);
-        const uint32_t y = (uint32_t)fminf(fmaxf((cy - lo.y) * scale.y, 0.0f), 1023.0f);
-        const uint32_t z = (uint32_t)fminf(fmaxf((cz - lo.z) * scale.z, 0.0f), 1023.0f);
-        codes[i] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+        const uint32_t x = (uint32_t)fminf(fmaxf((cx - lo.x) * scale.x, 0.0f), 2097151.0f);
+        const uint32_t y = (uint32_t)fminf(fmaxf((cy - lo.y) * scale.y, 0.0f), 2097151.0f);
+        const uint32_t z = (uint32_t)fminf(fmaxf((cz - lo.z) * scale.z, 0.0f), 2097151.0f);
+        codes[i] = (expand21(x) << 2) | (expand21(y) << 1) | expand21(z);
         idx[i] = i;
     }
 }
 
 // Common-prefix length of the keys at i and j (the index breaks ties), -1 out of range.
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ c, int n, int i, int j) {
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ c, int n, int i, int j) {
     if (j < 0 || j >= n) return -1;
-    const uint32_t a = c[i], b = c[j];
-    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(a ^ b);
+    const uint64_t a = c[i], b = c[j];
+    if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
+    return __clzll((long long)(a ^ b));
 }
 
 // Internal nodes 0 .. n-2; child ids < n-1 are internal nodes, id (n-1) + k is leaf k.
-__global__ void k_hierarchy(const uint32_t* __restrict__ c, int n, int2* child, int2* range, int* parent) {
+__global__ void k_hierarchy(const uint64_t* __restrict__ c, int n, int2* child, int2* range, int* parent) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
         const int d = (delta(c, n, i, i + 1) - delta(c, n, i, i - 1)) >= 0 ? 1 : -1;
         const int dmin = delta(c, n, i, i - d);
@@ -228,12 +232,12 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
             hi[a] = std::max(hi[a], c);
         }
     float scale[3];
-    for (int a = 0; a < 3; a++) scale[a] = hi[a] > lo[a] ? 1024.0f / (hi[a] - lo[a]) : 0.0f;
+    for (int a = 0; a < 3; a++) scale[a] = hi[a] > lo[a] ? 2097152.0f / (hi[a] - lo[a]) : 0.0f;
     const int N = (int)n;
     Bufs B;
     int rc;
-    if ((rc = B.dv.upload(verts, (size_t)n * 9 * 4)) || (rc = B.codes.alloc((size_t)n * 4)) ||
-        (rc = B.idx.alloc((size_t)n * 4)) || (rc = B.codes_s.alloc((size_t)n * 4)) || (rc = B.idx_s.alloc((size_t)n * 4)) ||
+    if ((rc = B.dv.upload(verts, (size_t)n * 9 * 4)) || (rc = B.codes.alloc((size_t)n * 8)) ||
+        (rc = B.idx.alloc((size_t)n * 4)) || (rc = B.codes_s.alloc((size_t)n * 8)) || (rc = B.idx_s.alloc((size_t)n * 4)) ||
         (rc = B.child.alloc((size_t)n * 8)) || (rc = B.range.alloc((size_t)n * 8)) ||
         (rc = B.parent.alloc((size_t)(2 * n) * 4)) || (rc = B.box.alloc((size_t)(2 * n) * sizeof(BoxF))) ||
         (rc = B.cost.alloc((size_t)(2 * n) * 4)) || (rc = B.collapse.alloc((size_t)n * 4)) ||
@@ -241,18 +245,18 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
         return rc;
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, 0, B.dv.as<float>(), (uint32_t)n,
-                       make_float3(lo[0], lo[1], lo[2]), make_float3(scale[0], scale[1], scale[2]), B.codes.as<uint32_t>(),
+                       make_float3(lo[0], lo[1], lo[2]), make_float3(scale[0], scale[1], scale[2]), B.codes.as<uint64_t>(),
                        B.idx.as<uint32_t>());
     SPK_CHECK_HIP(hipGetLastError());
     size_t tbytes = 0;
-    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, B.codes.as<uint32_t>(), B.codes_s.as<uint32_t>(),
-                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 30,
+    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
+                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 63,
                                                      (hipStream_t)0));
     if ((rc = B.tmp.alloc(std::max<size_t>(tbytes, 16)))) return rc;
-    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, tbytes, B.codes.as<uint32_t>(), B.codes_s.as<uint32_t>(),
-                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 30,
+    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
+                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 63,
                                                      (hipStream_t)0));
-    hipLaunchKernelGGL(k_hierarchy, dim3(blocks), dim3(256), 0, 0, B.codes_s.as<uint32_t>(), N, B.child.as<int2>(),
+    hipLaunchKernelGGL(k_hierarchy, dim3(blocks), dim3(256), 0, 0, B.codes_s.as<uint64_t>(), N, B.child.as<int2>(),
                        B.range.as<int2>(), B.parent.as<int>());
     SPK_CHECK_HIP(hipGetLastError());
     SPK_CHECK_HIP(hipMemset(B.flag.p, 0, (size_t)n * 4));
