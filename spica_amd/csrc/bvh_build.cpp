@@ -385,7 +385,7 @@ int collapse_bvh4(const std::vector<BvhNodeHost>& n2, std::vector<Bvh4NodeHost>&
 
 // ---- 8-bit child boxes ----------------------------------------------------------------
 // Per axis the node's box [lo, hi] (the union of its children) is split into 255 steps:
-// scale = (hi - lo) / 255, raised until code 255 reaches hi.  A child's lower plane takes the
+// scale = (hi - lo) / 255, raised (by doubling increments) until code 255 reaches hi.  A child's lower plane takes the
 // largest code whose decoding fmaf(q, scale, lo) is <= the plane, its upper plane the
 // smallest code whose decoding is >= it.  Device and host fmaf are both correctly rounded,
 // so the traversal decodes exactly the boxes checked here.  A flat axis has scale 0 and
@@ -406,7 +406,9 @@ void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHos
                 }
             if (lo > hi) lo = hi = 0.0f;  // no children (never for a built tree)
             float sc = (hi - lo) / 255.0f;
-            while (std::fmaf(255.0f, sc, lo) < hi) sc = std::nextafter(sc, FLT_MAX);
+            // raise the step until code 255 reaches hi; the increment doubles, so a nearly flat
+            // box far from the origin needs a few rounds, not one per ulp of the step
+            for (float add = std::nextafter(sc, FLT_MAX) - sc; std::fmaf(255.0f, sc, lo) < hi; add *= 2.0f) sc += add;
             q.origin[a] = lo;
             q.scale[a] = sc;
             uint32_t wlo = 0, whi = 0;
