@@ -45,7 +45,7 @@ const char* spk_version(void);
  * "where does render run" knob. */
 int spk_init(int device);
 
-/* Builds the BVH (binned SAH on the host, collapsed to 128-byte 4-wide nodes) and uploads
+/* Builds the BVH (binned SAH on the host, collapsed to 4-wide nodes of 64 bytes) and uploads
  * geometry, materials and lights.  Replaces BVHAccel::BVHAccel/construct
  * (sources/accelerators/bvh.cc:112-148) and Scene::Scene (sources/core/scene.cc:15-20),
  * together with the per-primitive material/light binding of
@@ -61,7 +61,7 @@ int spk_scene_destroy(spk_scene* scene);
 
 typedef struct spk_scene_stats {
     int64_t num_tris;
-    int64_t num_nodes;      /* interior 4-wide nodes (128 bytes each) */
+    int64_t num_nodes;      /* interior 4-wide nodes (64 bytes each, child boxes quantised) */
     int64_t num_leaves;
     int32_t max_depth;
     int32_t num_lights;
