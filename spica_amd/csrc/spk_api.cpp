@@ -268,7 +268,7 @@ struct spk_render {
     hipGraphExec_t pass_exec = nullptr;  // use_graph: one captured plain pass, replayed
     // learned proposal
     spk_guide* guide = nullptr;
-    DevBuf qg[4], rec[5];
+    DevBuf qg[4], rec[5], rec_s[5], rec_keys, rec_idx, rec_tmp;  // rec_s: the records sorted
     uint32_t rec_cap = 0;
     int64_t guide_trained = 0;   // training samples per pixel done since attaching
     int64_t train_passes = 0;
@@ -850,7 +850,8 @@ int spk_render_destroy(spk_render* r) {
                       &r->qs_o, &r->qs_d, &r->qs_c, &r->qm_o, &r->qm_d, &r->qm_c, &r->path_L, &r->film_sum,
                       &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch,
                       &r->qg[0], &r->qg[1], &r->qg[2], &r->qg[3], &r->rec[0], &r->rec[1], &r->rec[2], &r->rec[3],
-                      &r->rec[4]};
+                      &r->rec[4], &r->rec_s[0], &r->rec_s[1], &r->rec_s[2], &r->rec_s[3], &r->rec_s[4],
+                      &r->rec_keys, &r->rec_idx, &r->rec_tmp};
     for (DevBuf* b : bufs) b->release();
     delete r;
     return SPK_OK;
@@ -979,6 +980,16 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
             net.rec_keep = 1.0f / (float)k;
             if ((rc = run_passes(r, first, k, &net))) return rc;
             if (train) {
+                float4* in[5];
+                float4* out[5];
+                for (int q = 0; q < 5; q++) {
+                    in[q] = r->rec[q].as<float4>();
+                    out[q] = r->rec_s[q].as<float4>();
+                }
+                if ((rc = sort_guide_records(in, out, r->rec_cap, g->state.as<uint32_t>(), r->rec_keys.as<uint64_t>(),
+                                             r->rec_idx.as<uint32_t>(), r->rec_tmp.p, r->rec_tmp.bytes, r->stream)))
+                    return rc;
+                for (int q = 0; q < 5; q++) net.rec[q] = out[q];
                 const TrainBufs tb = g->train_bufs(g->bcap);
                 for (int st = 0; st < g->opts.steps_per_pass; st++) {
                     uint32_t h0 = g->opts.seed, h1 = (uint32_t)g->steps, h2 = 0x6a09e667u, h3 = 0xbb67ae85u;
@@ -1024,7 +1035,10 @@ int spk_render_set_guide(spk_render* r, spk_guide* g) {
     // training records of one round: about one kept vertex per pixel, room for four
     r->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)r->cfg.npix * 4u, uint64_t(1) << 24);
     for (int k = 0; k < 5; k++)
-        if ((rc = r->rec[k].alloc((size_t)r->rec_cap * 16))) return rc;
+        if ((rc = r->rec[k].alloc((size_t)r->rec_cap * 16)) || (rc = r->rec_s[k].alloc((size_t)r->rec_cap * 16))) return rc;
+    if ((rc = r->rec_keys.alloc((size_t)r->rec_cap * 16)) || (rc = r->rec_idx.alloc((size_t)r->rec_cap * 8)) ||
+        (rc = r->rec_tmp.alloc(std::max<size_t>(guide_record_sort_bytes(r->rec_cap), 16))))
+        return rc;
     r->bufs.qg[0] = r->qg[0].as<float4>();
     r->bufs.qg[1] = r->qg[1].as<float4>();
     r->bufs.qg[2] = r->qg[2].as<float4>();

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 
 #include "spk_guide.h"
@@ -167,7 +169,7 @@ __global__ void k_guide_batch(GuideNetDev net, const float4* __restrict__ path_L
             if (!(w > 0.0f) || !(w < 1.0e30f)) w = 0.0f;
             x0 = r0;
             x1 = make_float4(r1.x, r1.y, r1.z, w);
-            d = r2;
+            d = make_float4(r2.x, r2.y, r2.z, 0.0f);
         }
         tb.tb[0][j] = x0;
         tb.tb[1][j] = x1;
@@ -494,6 +496,65 @@ int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n
         hipLaunchKernelGGL(k_guide_eval<0>, dim3(grid), dim3(256), 0, st, wf, bias, n, x, aux, out_raw, res);
     else
         hipLaunchKernelGGL(k_guide_eval<1>, dim3(grid), dim3(256), 0, st, wf, bias, n, x, aux, out_raw, res);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+// ---- record order ----------------------------------------------------------------------------------
+// Key of record i: (sample, pixel, depth) -- 24, 32 and 8 bits -- for the rec_count stored
+// records, all ones after them, so the sort puts the stored records first in a fixed order
+// (a path vertex has one record).  Training rounds cover the first samples of a render, far
+// below 2^24.
+__global__ void k_record_keys(const float4* __restrict__ r2, const float4* __restrict__ r3,
+                              const float4* __restrict__ r4, uint32_t cap, const uint32_t* __restrict__ rec_count,
+                              uint64_t* keys, uint32_t* idx) {
+    const uint32_t n = min(*rec_count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+        keys[i] = i < n ? (((uint64_t)(__float_as_uint(r3[i].w) & 0xffffffu) << 40) |
+                           ((uint64_t)__float_as_uint(r4[i].w) << 8) | (uint64_t)(__float_as_uint(r2[i].w) & 255u))
+                        : ~0ull;
+        idx[i] = i;
+    }
+}
+
+struct RecordPtrs {
+    const float4* in[5];
+    float4* out[5];
+};
+
+__global__ void k_record_gather(RecordPtrs p, uint32_t cap, const uint32_t* __restrict__ rec_count,
+                                const uint32_t* __restrict__ order) {
+    const uint32_t n = min(*rec_count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t j = order[i];
+#pragma unroll
+        for (int k = 0; k < 5; k++) p.out[k][i] = p.in[k][j];
+    }
+}
+
+size_t guide_record_sort_bytes(uint32_t cap) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)cap, 0, 64);
+    return bytes;
+}
+
+int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap, const uint32_t* rec_count,
+                       uint64_t* keys, uint32_t* idx, void* tmp, size_t tmp_bytes, hipStream_t st) {
+    if (cap == 0) return SPK_OK;
+    const uint32_t blocks = std::min<uint32_t>((cap + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_record_keys, dim3(blocks), dim3(256), 0, st, (const float4*)rec[2], (const float4*)rec[3],
+                       (const float4*)rec[4], cap, rec_count, keys, idx);
+    SPK_CHECK_HIP(hipGetLastError());
+    size_t bytes = tmp_bytes;
+    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, (const uint64_t*)keys, keys + cap, (const uint32_t*)idx,
+                                                     idx + cap, (int)cap, 0, 64, st));
+    RecordPtrs p;
+    for (int k = 0; k < 5; k++) {
+        p.in[k] = rec[k];
+        p.out[k] = out[k];
+    }
+    hipLaunchKernelGGL(k_record_gather, dim3(blocks), dim3(256), 0, st, p, cap, rec_count, (const uint32_t*)(idx + cap));
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
 }

@@ -182,6 +182,12 @@ constexpr uint32_t kGuideColsPerSplit = 1024;
 int launch_guide_images(const float* params, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t st);
 int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n, const float4* x, const float4* aux,
                       float* out_raw, float4* res, int grid, hipStream_t st);
+// Training records of one round in a fixed order: sorted by (sample, pixel, depth) into out[0..4], so
+// the batches, and with them the trained network, do not depend on the order in which the
+// queue appends stored the records.  keys/idx hold 2 * cap entries each (input and sorted).
+size_t guide_record_sort_bytes(uint32_t cap);
+int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap, const uint32_t* rec_count,
+                       uint64_t* keys, uint32_t* idx, void* tmp, size_t tmp_bytes, hipStream_t st);
 int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net, const float4* path_L,
                       const AdamCfg& cfg, uint32_t seed, uint32_t direct_n, bool bump_t, float* grad_out,
                       hipStream_t st);

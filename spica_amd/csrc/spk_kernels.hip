@@ -1837,7 +1837,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         wave_lds_sync();
         bool pushA = false, pushR = false;
         float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0), beta = mk(0, 0, 0);
-        uint32_t nA_flags = 0, path = 0, pixel = 0;
+        uint32_t nA_flags = 0, path = 0, pixel = 0, rsample = 0;
         if (active) {
             const float* o = ob + lane * kGuideObStride;
             GuideMix mix;
@@ -1847,6 +1847,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             beta = xyz(G2);
             const uint32_t flags = __float_as_uint(G1.w);
             const uint32_t sample = sample_base + (flags >> 7);
+            rsample = sample;
             const uint32_t mbits = __float_as_uint(NG.w);
             const float3 ng = xyz(NG), p = xyz(G0), wo = xyz(G1);
             Bsdf b;
@@ -1914,9 +1915,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                 const float4 L = rb.path_L[path];
                 net.rec[0][sr] = make_float4(x[0], x[1], x[2], x[3]);
                 net.rec[1][sr] = make_float4(x[4], x[5], x[6], __uint_as_float(path));
-                net.rec[2][sr] = make_float4(nA_d.x, nA_d.y, nA_d.z, 0.0f);
-                net.rec[3][sr] = make_float4(L.x, L.y, L.z, 0.0f);
-                net.rec[4][sr] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+                net.rec[2][sr] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float((uint32_t)depth));
+                net.rec[3][sr] = make_float4(L.x, L.y, L.z, __uint_as_float(rsample));
+                net.rec[4][sr] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(pixel));
             }
         }
         wave_lds_sync();

@@ -197,3 +197,25 @@ def test_guided_render_reduces_variance(gpu):
     # about 1 training record per pixel and round: the gain grows with the frame (3x at 1280x720)
     assert out[False] / out[True] > 1.5
     gs.close()
+
+
+def test_guided_render_is_deterministic(gpu):
+    """Two guided renders with the same settings give the same film bit for bit: the training
+    records are sorted by (sample, pixel, depth) before every round's Adam steps, so neither
+    the batches nor the trained network depend on the order of the queue appends."""
+    from spica_amd import configs, render
+    wl = configs.c3_occluded_glossy(width=160, height=90)
+    gs = render.GpuScene(wl.scene)
+    films = []
+    for _ in range(2):
+        gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=96), queue_size=1 << 22, block_size=64)
+        g = render.Guide(batch_size=1 << 14, train_spp=32, train_pass_spp=4, steps_per_pass=2)
+        gr.set_guide(g)
+        gr.run(96)
+        assert gr.stats()["train_passes"] == 8
+        films.append(gr.film()[0])
+        gr.close()
+        g.close()
+    print(f"guided film sums {films[0].sum():.9g} {films[1].sum():.9g}")
+    assert np.array_equal(films[0], films[1])
+    gs.close()
