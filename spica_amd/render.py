@@ -10,15 +10,12 @@ import numpy as np
 
 from . import _abi
 
-_initialised = {}
-
-
 def init(device=0):
+    """Loads the library and makes `device` current for the calling thread (the library, like
+    HIP, keeps the current device per thread; the call is cheap, so scenes may be built from
+    several threads)."""
     lib = _abi.load()
-    if _initialised.get(device):
-        return lib
     _abi.check(lib.spk_init(device), "spk_init")
-    _initialised[device] = True
     return lib
 
 
