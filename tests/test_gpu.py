@@ -177,6 +177,29 @@ def test_lbvh_and_sah_agree(gpu):
     assert diff <= n * 1e-4
 
 
+def test_scenes_built_on_several_threads(gpu):
+    """Scenes built side by side on host threads (the library keeps the current device per
+    thread) trace exactly as the same scenes built one after another."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+
+    scenes = [configs.c1_spheres(seed=s, n_spheres=64).scene for s in range(6)]
+    rng = np.random.default_rng(3)
+    n = 200_000
+    o = torch.from_numpy(rng.uniform(-6, 6, size=(n, 3)).astype(np.float32)).cuda()
+    d = torch.from_numpy(rng.normal(size=(n, 3)).astype(np.float32)).cuda()
+    for builder in ("sah", "lbvh"):
+        seq = [gpu.GpuScene(sc, builder=builder) for sc in scenes]
+        with ThreadPoolExecutor(max_workers=4) as ex:
+            par = list(ex.map(lambda sc: gpu.GpuScene(sc, builder=builder), scenes))
+        for a, b in zip(seq, par):
+            assert a.stats()["num_nodes"] == b.stats()["num_nodes"]
+            (pa, ta), (pb, tb) = a.trace(o, d), b.trace(o, d)
+            assert torch.equal(pa, pb) and torch.equal(ta, tb)
+        for g in seq + par:
+            g.close()
+
+
 def test_deep_stack_spill_parity(gpu, oracle):
     """Forty concentric spheres seen from their centre: every level's child boxes contain
     the origin, so a ray's traversal stack outgrows its 16 LDS slots and spills to memory;
