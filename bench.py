@@ -49,20 +49,36 @@ def read_traffic(profile_dir, rays_per_launch):
     return round(d["bytes_per_ray"] * rays_per_launch)
 
 
-def cpu_baseline(w, threads, budget_s):
-    """The CPU checker (spica's estimator, FP64, spica's BVH) on a bounded pixel sample."""
+def host_cpus():
+    """What this host gives the process: CPU model, logical CPUs, the affinity set and the
+    cgroup CPU quota (cores' worth of time, None when unlimited)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    return dict(model=model, logical_cpus=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), cgroup_quota_cpus=quota)
+
+
+def cpu_rate(sc, w, threads, budget_s, seed):
+    """Rays and paths per second of the CPU checker over random pixels of w, ~budget_s seconds."""
     import numpy as np
 
-    from oracle.oracle import Oracle
-
-    orc = Oracle()
-    t0 = time.perf_counter()
-    sc = orc.scene(w.scene)
-    build_s = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(seed)
     npix = w.camera.width * w.camera.height
-    # calibrate on a small sample, then run the sample that fits the budget
-    n = 2000
+    n = 2000 * max(1, threads // 4)
     rays = paths = 0
     elapsed = 0.0
     while elapsed < budget_s:
@@ -74,19 +90,41 @@ def cpu_baseline(w, threads, budget_s):
         elapsed += dt
         rays += int(st[0])
         paths += n
-        n = int(min(200000, max(2000, n * max(1.0, 0.5 * (budget_s - elapsed) / max(dt, 1e-3)))))
+        n = int(min(1000000, max(2000, n * max(1.0, 0.5 * (budget_s - elapsed) / max(dt, 1e-3)))))
+    return rays, paths, elapsed
+
+
+def cpu_baseline(w, threads, budget_s):
+    """The CPU checker (spica's estimator, FP64, spica's BVH) on a bounded pixel sample, on
+    every CPU the process may use, plus one thread for the per-core rate; spica's own compiled
+    path runs at the calibrated ratio of the checker's speed (oracle/calibration.json)."""
+    from oracle.oracle import Oracle
+
+    orc = Oracle()
+    t0 = time.perf_counter()
+    sc = orc.scene(w.scene)
+    build_s = time.perf_counter() - t0
+    rays, paths, elapsed = cpu_rate(sc, w, threads, budget_s, 0)
+    r1, p1, e1 = cpu_rate(sc, w, 1, min(5.0, budget_s), 1)
+    value = rays / elapsed / 1e6
+    one = r1 / e1 / 1e6
     cal = None
     cal_path = os.path.join(ROOT, "oracle", "calibration.json")
     if os.path.exists(cal_path):
         with open(cal_path) as f:
             c = json.load(f)
-        r = c["spica_over_restatement_1thread"]
-        cal = dict(spica_over_port=r, spica_equivalent_mrays_per_s=round(r * rays / elapsed / 1e6, 3),
-                   measured=c["workload"] + "; " + c["measured"])
-    return dict(value=rays / elapsed / 1e6, unit="Mrays/s", cores=threads, kind="port", calibration=cal,
-                sample=f"{paths} random pixels of the 1920x1080 frame at 1 spp ({rays} rays, {elapsed:.1f} s); "
-                       f"spica-style BVH build {build_s:.1f} s excluded",
-                mpaths_per_s=paths / elapsed / 1e6)
+        r = c["ratios"]["c1"]["ratio"]
+        cal = dict(spica_over_port=r, spica_over_port_c0=c["ratios"]["c0"]["ratio"],
+                   spica_equivalent_mrays_per_s=round(r * value, 3),
+                   measured=f"spica's compiled path vs the checker, one thread each, C1 ({c['ratios']['c1']['paths']} "
+                            f"camera paths) and C0 ({c['ratios']['c0']['paths']}); {c['host']}")
+    return dict(value=round(value, 3), unit="Mrays/s", cores=threads, kind="port", calibration=cal,
+                host=host_cpus(), one_thread_mrays_per_s=round(one, 4),
+                scaling_over_one_thread=round(value / one, 2),
+                sample=f"{paths} random pixels of the 1920x1080 frame at 1 spp ({rays} rays, {elapsed:.1f} s) on "
+                       f"{threads} threads; one thread: {p1} pixels, {e1:.1f} s; spica-style BVH build "
+                       f"{build_s:.1f} s excluded",
+                mpaths_per_s=round(paths / elapsed / 1e6, 4))
 
 
 def load_reference(name):
@@ -164,7 +202,8 @@ def main():
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="CPU-baseline threads (default: every CPU this process may run on)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queue", type=int, default=0, help="paths in flight (0 = autotune)")
     ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
@@ -292,7 +331,7 @@ def main():
         scenes = build_scenes([sc for sc, _, _, _ in wb.items])
         scene_s = time.perf_counter() - t0
         tris = sum(g.stats()["num_tris"] for g in scenes)
-        bvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3  # summed over scenes built side by side
+        bvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3  # summed over builds running side by side on 4 threads
         setup_s = gen_s + scene_s
         items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
         b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
@@ -311,7 +350,7 @@ def main():
                      images=len(items), triangles=int(tris), paths=int(info["paths"]), passes=int(info["passes"]),
                      seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
                      mpaths_per_s=round(info["paths"] / bdt / 1e6, 3), setup_s=round(setup_s, 1),
-                     generate_s=round(gen_s, 1), scenes_s=round(scene_s, 1), bvh_build_s=round(bvh_s, 1),
+                     generate_s=round(gen_s, 1), scenes_s=round(scene_s, 1), bvh_build_thread_s=round(bvh_s, 1),
                      queue=int(info["queue"]), block=args.block)
         c2_ref = load_batch_reference()
         if c2_ref is not None:
@@ -344,7 +383,7 @@ def main():
         ldt = time.perf_counter() - t0
         st1 = b.stats()
         lrays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
-        batch["lbvh"] = dict(scenes_s=round(lscene_s, 1), bvh_build_s=round(lbvh_s, 1), seconds=round(ldt, 3),
+        batch["lbvh"] = dict(scenes_s=round(lscene_s, 1), bvh_build_thread_s=round(lbvh_s, 1), seconds=round(ldt, 3),
                              mrays_per_s=round(lrays / ldt / 1e6, 3))
         log(f"bench: batch {batch}")
         b.close()

@@ -19,16 +19,36 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17", "-Wa
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build_lib(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def build_lib(verbose=False, jobs=8):
+    """Compiles every source to an object (in parallel; always, so a header change is never
+    missed), then links libspk.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, src + ".o")
+        cmd = [HIPCC] + compile_flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {src}")
+        return obj
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libspk.so")
+        raise RuntimeError("hipcc failed linking libspk.so")
     os.replace(tmp, LIB)
     return LIB
 

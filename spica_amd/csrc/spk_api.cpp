@@ -49,7 +49,6 @@ void DevBuf::release() {
     bytes = 0;
 }
 
-int trace_blocks_per_cu(int block);
 int zero_u32(uint32_t* p, int n, hipStream_t st);
 int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
                    uint32_t* out, hipStream_t st);
@@ -272,6 +271,17 @@ struct spk_render {
     uint32_t rec_cap = 0;
     int64_t guide_trained = 0;   // training samples per pixel done since attaching
     int64_t train_passes = 0;
+    // Device buffers free themselves after this body; the stream, events and graph are
+    // released here, so an early error return from spk_render_create leaks nothing.
+    ~spk_render() {
+        (void)hipSetDevice(device);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (pass_exec) (void)hipGraphExecDestroy(pass_exec);
+    }
 };
 
 extern "C" {
@@ -368,6 +378,15 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     }
     if (d->env_width > 0 && (d->env_height <= 0 || !d->env_rgb)) {
         set_error("spk_scene_create: bad environment map");
+        return SPK_ERR_INVALID;
+    }
+    // spica's light keeps the transpose of toWorld and inverts that whole 4x4 matrix
+    // (envmap.cc:19, light.cc:6-10); for an affine toWorld the transpose's last column is
+    // (0, 0, 0, 1), so the direction part of that inverse is the inverse of the transposed 3x3,
+    // which is what the kernels apply.  A projective toWorld would break that identity.
+    if (d->env_width > 0 && (d->env_to_world[12] != 0.0 || d->env_to_world[13] != 0.0 ||
+                             d->env_to_world[14] != 0.0 || d->env_to_world[15] != 1.0)) {
+        set_error("spk_scene_create: environment toWorld must be affine (last row 0 0 0 1)");
         return SPK_ERR_INVALID;
     }
 
@@ -665,6 +684,16 @@ int spk_scene_destroy(spk_scene* s) {
     return SPK_OK;
 }
 
+}  // extern "C"
+
+void spk::scene_trace_info(const spk_scene* s, SceneTraceInfo* out) {
+    out->dev = &s->dev;
+    out->device = s->device;
+    out->spill_cap = s->spill_cap;
+}
+
+extern "C" {
+
 int spk_scene_stats_get(const spk_scene* s, spk_scene_stats* out) {
     clear_error();
     if (!s || !out) {
@@ -838,21 +867,6 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
 
 int spk_render_destroy(spk_render* r) {
     clear_error();
-    if (!r) return SPK_OK;
-    (void)hipSetDevice(r->device);
-    if (r->stream) {
-        (void)hipStreamSynchronize(r->stream);
-        (void)hipStreamDestroy(r->stream);
-    }
-    for (hipEvent_t e : r->events) (void)hipEventDestroy(e);
-    if (r->pass_exec) (void)hipGraphExecDestroy(r->pass_exec);
-    DevBuf* bufs[] = {&r->qa_o[0], &r->qa_o[1], &r->qa_d[0], &r->qa_d[1], &r->qa_b[0], &r->qa_b[1], &r->hit,
-                      &r->qs_o, &r->qs_d, &r->qs_c, &r->qm_o, &r->qm_d, &r->qm_c, &r->path_L, &r->film_sum,
-                      &r->film_c, &r->film_sq, &r->film_sqc, &r->spill, &r->cfg_dev, &r->ps_dev, &r->scratch,
-                      &r->qg[0], &r->qg[1], &r->qg[2], &r->qg[3], &r->rec[0], &r->rec[1], &r->rec[2], &r->rec[3],
-                      &r->rec[4], &r->rec_s[0], &r->rec_s[1], &r->rec_s[2], &r->rec_s[3], &r->rec_s[4],
-                      &r->rec_keys, &r->rec_idx, &r->rec_tmp};
-    for (DevBuf* b : bufs) b->release();
     delete r;
     return SPK_OK;
 }
@@ -876,7 +890,8 @@ GuideNetDev guide_net(spk_render* r, bool record) {
     n.record = record ? 1 : 0;
     for (int k = 0; k < 5; k++) n.rec[k] = r->rec[k].as<float4>();
     n.rec_cap = r->rec_cap;
-    n.rec_count = g->state.as<uint32_t>();  // GuideState::rec_count, its first word
+    n.rec_count = g->state.as<uint32_t>();      // GuideState::rec_count, its first word
+    n.rec_fin = g->state.as<uint32_t>() + 1;    // GuideState::rec_fin, the second
     return n;
 }
 
@@ -974,8 +989,8 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
             const int32_t k = train ? (int32_t)std::min<int64_t>({(int64_t)left, (int64_t)g->opts.train_pass_spp,
                                                                   g->opts.train_spp - r->guide_trained})
                                     : left;
-            // rec_count is the first word of the guide state
-            if (train && (rc = zero_u32(g->state.as<uint32_t>(), 1, r->stream))) return rc;
+            // rec_count and rec_fin are the first two words of the guide state
+            if (train && (rc = zero_u32(g->state.as<uint32_t>(), 2, r->stream))) return rc;
             GuideNetDev net = guide_net(r, train);
             net.rec_keep = 1.0f / (float)k;
             if ((rc = run_passes(r, first, k, &net))) return rc;
@@ -994,7 +1009,7 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
                 for (int st = 0; st < g->opts.steps_per_pass; st++) {
                     uint32_t h0 = g->opts.seed, h1 = (uint32_t)g->steps, h2 = 0x6a09e667u, h3 = 0xbb67ae85u;
                     pcg4d(h0, h1, h2, h3);
-                    if ((rc = launch_guide_step(g->bufs(), tb, &net, r->path_L.as<float4>(), g->adam(), h0, 0u, true,
+                    if ((rc = launch_guide_step(g->bufs(), tb, &net, g->adam(), h0, 0u, true,
                                                 nullptr, r->stream)))
                         return rc;
                     g->steps++;
@@ -1207,6 +1222,11 @@ struct spk_batch {
     int64_t film_pixels = 0;
     int graph = 0;
     hipGraphExec_t exec = nullptr;    // one pass (graph = 1) or the whole batch (graph = 2)
+    ~spk_batch() {
+        if (r) (void)hipSetDevice(r->device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (r) spk_render_destroy(r);
+    }
 };
 
 namespace {
@@ -1356,10 +1376,8 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
     if ((rc = b->scenes_dev.upload(sd.data(), sd.size() * sizeof(SceneDev))) ||
         (rc = b->images_dev.upload(b->images.data(), b->images.size() * sizeof(ImageDev))) ||
         (rc = b->chunks_dev.upload(b->chunks.data(), b->chunks.size() * sizeof(ChunkDev))) ||
-        (rc = b->passes_dev.upload(b->passes.data(), b->passes.size() * sizeof(PassDev)))) {
-        spk_render_destroy(r);
+        (rc = b->passes_dev.upload(b->passes.data(), b->passes.size() * sizeof(PassDev))))
         return rc;
-    }
     r->bufs.scenes = b->scenes_dev.as<SceneDev>();
     r->bufs.images = b->images_dev.as<ImageDev>();
     r->bufs.chunks = b->chunks_dev.as<ChunkDev>();
@@ -1385,12 +1403,6 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
 
 int spk_batch_destroy(spk_batch* b) {
     clear_error();
-    if (!b) return SPK_OK;
-    if (b->r) (void)hipSetDevice(b->r->device);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    DevBuf* bufs[] = {&b->scenes_dev, &b->images_dev, &b->chunks_dev, &b->passes_dev};
-    for (DevBuf* x : bufs) x->release();
-    if (b->r) spk_render_destroy(b->r);
     delete b;
     return SPK_OK;
 }

@@ -305,20 +305,21 @@ struct GuideNetDev {
     float frac;            // probability of sampling the guide at a vertex
     int record;            // 1: store training records in this pass
     float rec_keep;        // fraction of continuing vertices recorded
-    float4* rec[5];        // (x0..x3) (x4..x6, path) (w_i) (L before) (beta before)
+    float4* rec[5];        // (x0..x3) (x4..x6, path) (w_i) (L before, then L gathered after) (beta before)
     uint32_t rec_cap;
     uint32_t* rec_count;
+    uint32_t* rec_fin;     // records whose gathered radiance is final (the passes before this one)
 };
 
 struct GuideState {
-    uint32_t rec_count;    // records appended in the current training pass (may exceed capacity)
+    uint32_t rec_count;    // records appended in the current training round (may exceed capacity)
+    uint32_t rec_fin;      // records of the round's finished passes: rec[3] holds their gathered radiance
     uint32_t n_rec;        // usable records
     uint32_t b_eff;        // samples in the current batch
     uint32_t active;       // the current step runs
     uint32_t t;            // Adam steps taken
     uint32_t perm_a;
     uint32_t perm_off;
-    uint32_t pad;
     float loss_sum;        // sum over the batch of -(F/p) log q
     float weight_sum;
     uint32_t pad2[2];

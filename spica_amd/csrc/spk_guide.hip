@@ -2,7 +2,7 @@
 //
 // One Adam step over a batch of B samples (x, w_i, F/p):
 //   k_guide_step_begin  batch size and record permutation for this step (1 thread)
-//   k_guide_batch       training records of the last pass -> batch, with F/p from the path radiance
+//   k_guide_batch       training records of the last round -> batch, F/p from their gathered radiance
 //   k_guide_fb          fused forward + backward-data per 16-sample tile: stores the one-blob code,
 //                       hidden activations and the back-propagated deltas (bf16, unit-major)
 //   k_guide_dw          weight gradients dW = delta * a^T as MFMA tiles over sample ranges
@@ -150,8 +150,7 @@ __global__ void k_guide_step_begin(GuideState* gs, uint32_t batch, uint32_t rec_
 
 __device__ __forceinline__ float safe_ratio(float a, float b) { return b > 0.0f ? a / b : 0.0f; }
 
-__global__ void k_guide_batch(GuideNetDev net, const float4* __restrict__ path_L, const GuideState* __restrict__ gs,
-                              TrainBufs tb) {
+__global__ void k_guide_batch(GuideNetDev net, const GuideState* __restrict__ gs, TrainBufs tb) {
     if (!gs->active) return;
     const uint32_t b_eff = gs->b_eff, n = gs->n_rec, a = gs->perm_a, off = gs->perm_off;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < tb.ncols; j += gridDim.x * blockDim.x) {
@@ -161,11 +160,10 @@ __global__ void k_guide_batch(GuideNetDev net, const float4* __restrict__ path_L
             const uint32_t i = (uint32_t)(((uint64_t)j * a + off) % n);
             const float4 r0 = net.rec[0][i], r1 = net.rec[1][i], r2 = net.rec[2][i], r3 = net.rec[3][i],
                          r4 = net.rec[4][i];
-            const float4 L = path_L[__float_as_uint(r1.w)];
-            // F/p of the sampled direction: the radiance the path gathered after this vertex,
-            // divided by the throughput that reached it.
-            float w = 0.2126f * safe_ratio(L.x - r3.x, r4.x) + 0.7152f * safe_ratio(L.y - r3.y, r4.y) +
-                      0.0722f * safe_ratio(L.z - r3.z, r4.z);
+            // F/p of the sampled direction: the radiance the path gathered after this vertex
+            // (stored in the record when its pass ended), divided by the throughput that reached it.
+            float w = 0.2126f * safe_ratio(r3.x, r4.x) + 0.7152f * safe_ratio(r3.y, r4.y) +
+                      0.0722f * safe_ratio(r3.z, r4.z);
             if (!(w > 0.0f) || !(w < 1.0e30f)) w = 0.0f;
             x0 = r0;
             x1 = make_float4(r1.x, r1.y, r1.z, w);
@@ -559,12 +557,12 @@ int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap,
     return SPK_OK;
 }
 
-int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net, const float4* path_L,
+int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net,
                       const AdamCfg& cfg, uint32_t seed, uint32_t direct_n, bool bump_t, float* grad_out,
                       hipStream_t st) {
     hipLaunchKernelGGL(k_guide_step_begin, dim3(1), dim3(64), 0, st, gb.gs, tb.batch, net ? net->rec_cap : 0u, seed,
                        direct_n, bump_t ? 1u : 0u);
-    if (!direct_n) hipLaunchKernelGGL(k_guide_batch, dim3(1024), dim3(256), 0, st, *net, path_L, (const GuideState*)gb.gs, tb);
+    if (!direct_n) hipLaunchKernelGGL(k_guide_batch, dim3(1024), dim3(256), 0, st, *net, (const GuideState*)gb.gs, tb);
     const uint32_t fb_blocks = std::max(1u, std::min((uint32_t)gb.grid, (tb.ncols / 16 + 3) / 4));
     hipLaunchKernelGGL(k_guide_fb, dim3(fb_blocks), dim3(256), 0, st, (const uint16_t*)gb.wf, (const float*)gb.bias,
                        (const uint16_t*)gb.wb, tb, gb.gs);

@@ -305,7 +305,7 @@ static int direct_steps(spk_guide* g, int64_t n, const float* x, const float* di
     if ((rc = upload_batch(g, n, x, dirs, weight, ncols))) return rc;
     const TrainBufs tb = g->train_bufs(ncols);
     for (int s = 0; s < steps; s++) {
-        if ((rc = launch_guide_step(g->bufs(), tb, nullptr, nullptr, g->adam(), (uint32_t)g->steps, (uint32_t)n,
+        if ((rc = launch_guide_step(g->bufs(), tb, nullptr, g->adam(), (uint32_t)g->steps, (uint32_t)n,
                                     grad == nullptr, grad ? g->grad.as<float>() : nullptr, g->stream)))
             return rc;
         if (!grad) g->steps++;

@@ -80,9 +80,30 @@ void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHos
 int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhResult& res);
 
 // ---- device buffers -----------------------------------------------------------
+// One device allocation, freed when the owner goes out of scope (or on release()), so early
+// error returns and temporaries do not leak.  Not copyable; moving hands the allocation over.
+// The owner keeps the allocation's device current when it is freed.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     int alloc(size_t nbytes);
     int upload(const void* src, size_t nbytes);
     void release();
@@ -102,6 +123,14 @@ struct TraceBufs {
 };
 int launch_trace(const SceneDev& s, const TraceBufs& b, uint32_t* work_counter, uint32_t* spill, int spill_cap,
                  int grid, hipStream_t st);
+int trace_blocks_per_cu(int block);
+// What a trace needs from a scene handle (the handle's layout is private to spk_api.cpp).
+struct SceneTraceInfo {
+    const SceneDev* dev;
+    int device;
+    int spill_cap;      // traversal stack entries a thread may spill past its LDS slots
+};
+void scene_trace_info(const spk_scene* s, SceneTraceInfo* out);
 
 struct RenderBufs {
     // extend queues (ping-pong): origin|path, dir|flags, beta
@@ -188,7 +217,7 @@ int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n
 size_t guide_record_sort_bytes(uint32_t cap);
 int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap, const uint32_t* rec_count,
                        uint64_t* keys, uint32_t* idx, void* tmp, size_t tmp_bytes, hipStream_t st);
-int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net, const float4* path_L,
+int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net,
                       const AdamCfg& cfg, uint32_t seed, uint32_t direct_n, bool bump_t, float* grad_out,
                       hipStream_t st);
 

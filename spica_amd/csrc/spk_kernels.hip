@@ -1886,7 +1886,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                     if (cont) {
                         pushA = true;
                         if (net.record) {
-                            uint32_t h0 = path, h1 = (uint32_t)depth, h2 = sample, h3 = 0x7f4a7c15u;
+                            // keyed by (pixel, depth, sample), not the pass-local path slot, so the
+                            // kept records do not depend on how the samples are cut into passes
+                            uint32_t h0 = pixel, h1 = (uint32_t)depth, h2 = sample, h3 = 0x7f4a7c15u;
                             pcg4d(h0, h1, h2, h3);
                             pushR = u32_to_unit(h0) < net.rec_keep;
                         }
@@ -1922,6 +1924,22 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         }
         wave_lds_sync();
     }
+}
+
+// Training records of the pass that just ended: rec[3] held the path's radiance when the
+// vertex was recorded; once the path has ended it becomes the radiance gathered after the
+// vertex, L_end - L_before.  This runs before the next pass's camera kernel reuses the path
+// slots, so a training round that spans several passes trains on its own paths' radiance.
+__global__ __launch_bounds__(256) void k_guide_finalize(GuideNetDev net, const float4* __restrict__ path_L) {
+    const uint32_t lo = *net.rec_fin, hi = min(*net.rec_count, net.rec_cap);
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        const float4 r3 = net.rec[3][i];
+        const float4 L = path_L[__float_as_uint(net.rec[1][i].w)];
+        net.rec[3][i] = make_float4(L.x - r3.x, L.y - r3.y, L.z - r3.z, r3.w);
+    }
+}
+__global__ void k_guide_finalize_mark(GuideNetDev net) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *net.rec_fin = min(*net.rec_count, net.rec_cap);
 }
 
 // ---- film: Kahan-compensated accumulation (film.cc:65-74) -------------------------------------------
@@ -2175,6 +2193,10 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
         }
     }
     hipLaunchKernelGGL(k_film<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
+    if (!BATCH && guide && guide->record) {
+        hipLaunchKernelGGL(k_guide_finalize, dim3(ls.grid_stream), dim3(256), 0, st, *guide, (const float4*)b.path_L);
+        hipLaunchKernelGGL(k_guide_finalize_mark, dim3(1), dim3(64), 0, st, *guide);
+    }
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
 }

@@ -117,7 +117,7 @@ void Film::saveHDR(const std::string& path) const {
     std::vector<unsigned char> out;
     char head[160];
     snprintf(head, sizeof(head),
-             "#?RADIANCE\n# written by spica_amd\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n-Y %d +X %d\n", h_,
+             "#?RADIANCE\n# Made with 100%% pure HDR Shop\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n-Y %d +X %d\n", h_,
              w_);
     out.insert(out.end(), head, head + strlen(head));
     std::vector<unsigned char> px((size_t)w_ * 4);

@@ -102,7 +102,7 @@ def write_rgbe(path, rgb):
     enc = np.zeros((h, w, 4), dtype=np.uint8)
     enc[..., :3] = np.clip(rgb * sc[..., None], 0, 255).astype(np.uint8)
     enc[..., 3] = np.where(ok, e + 128, 0).astype(np.uint8)
-    out = bytearray(b"#?RADIANCE\n# written by spica_amd\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n")
+    out = bytearray(b"#?RADIANCE\n# Made with 100% pure HDR Shop\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n")
     out += f"-Y {h} +X {w}\n".encode("ascii")
     for y in range(h):
         out += bytes([2, 2, (w >> 8) & 0xFF, w & 0xFF])

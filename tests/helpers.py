@@ -83,7 +83,7 @@ def tri_intersect64(v, org, dirv):
     return t, u, w, det
 
 
-def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6, edge=1e-4):
+def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6, edge=1e-6):
     """Mismatched closest-hit ids, classified:
     near_tie  both triangles hit (float64) at t within rel_t of each other;
     edge      the disagreement is explained by a hit within the float32 reach of a
@@ -135,15 +135,39 @@ def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6
     return out
 
 
+# Relative precision of a float32 film mean against a float64 one: per-sample radiance in
+# float32 carries a few ulp (1e-7) per bounce, the Kahan film sums keep the mean to ~1e-7.
+# A pixel whose samples are all equal (an emitter seen directly, a black background) has
+# zero sample variance on both sides, and its means differ only by this precision; the
+# floor keeps z finite there and is far below any statistical standard error.
+FILM_REL_PRECISION = 1e-5
+
+
 def zstats(sum_a, sq_a, n_a, sum_b, sq_b, n_b):
-    """Per pixel-channel z of the difference of two sample means (Welch)."""
+    """Per pixel-channel z of the difference of two sample means (Welch), with the float
+    precision floor added to the standard error."""
     ma = sum_a / n_a
     mb = sum_b / n_b
     va = np.maximum(sq_a / n_a - ma * ma, 0.0) * n_a / (n_a - 1)
     vb = np.maximum(sq_b / n_b - mb * mb, 0.0) * n_b / (n_b - 1)
-    se = np.sqrt(va / n_a + vb / n_b)
+    floor = FILM_REL_PRECISION * np.maximum(np.abs(ma), np.abs(mb))
+    se = np.sqrt(va / n_a + vb / n_b + floor * floor)
     z = np.where(se > 0, (ma - mb) / np.where(se > 0, se, 1.0), np.where(np.abs(ma - mb) > 0, np.inf, 0.0))
     return z, ma, mb
+
+
+def worst_pixels(z, sum_a, sq_a, n_a, sum_b, sq_b, n_b, k=5):
+    """The k largest |z| pixel-channels with both sides' mean, sample deviation and count."""
+    flat = np.argsort(-np.abs(z).reshape(-1))[:k]
+    out = []
+    for f in flat:
+        p, c = divmod(int(f), z.shape[1])
+        na, nb = float(np.broadcast_to(n_a, z.shape)[p, c]), float(np.broadcast_to(n_b, z.shape)[p, c])
+        ma, mb = sum_a[p, c] / na, sum_b[p, c] / nb
+        sa = np.sqrt(max(sq_a[p, c] / na - ma * ma, 0.0))
+        sb = np.sqrt(max(sq_b[p, c] / nb - mb * mb, 0.0))
+        out.append(f"px {p} ch {c}: z {z[p, c]:.2f} a {ma:.6g} (sd {sa:.4g}, n {na:.0f}) b {mb:.6g} (sd {sb:.4g}, n {nb:.0f})")
+    return out
 
 
 def furnace_room(albedo=0.5, Le=1.0, size=1.0):

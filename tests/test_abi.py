@@ -50,6 +50,13 @@ def test_bad_descriptors_are_rejected():
     d.light_ids = lid.ctypes.data_as(_abi.c_i32_p)
     rc = lib.spk_scene_create(ctypes.byref(d), ctypes.byref(h))
     assert rc == _abi.SPK_ERR_INVALID and b"light" in lib.spk_last_error()
+    s = configs.cornell_box().scene
+    proj = np.eye(4)
+    proj[3, 0] = 0.5  # a projective environment transform
+    s.set_envmap(np.ones((4, 8, 3), np.float32), to_world=proj)
+    d = s.desc()
+    rc = lib.spk_scene_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _abi.SPK_ERR_INVALID and b"affine" in lib.spk_last_error()
     assert lib.spk_scene_create(None, ctypes.byref(h)) == _abi.SPK_ERR_INVALID
     assert lib.spk_render_run(None, 1, 0) == _abi.SPK_ERR_INVALID
     assert lib.spk_sampler_u32(7, 0, 0, None, None, None, None, 0) == _abi.SPK_ERR_INVALID

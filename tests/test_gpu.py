@@ -2,8 +2,9 @@
 
 Closest hits: primitive ids equal the float64 path's on seeded ray sets, except
 mismatches classified as near-ties (both triangles hit, |dt| <= 1e-6 t) or edge
-cases (a barycentric coordinate within 1e-4 of an edge, where float32 and float64
-legitimately disagree); no other class is allowed.
+cases (a barycentric coordinate within the float32 reach of an edge -- a few float32
+rounding steps of the hit position in triangle sizes, at least 1e-6 -- where float32
+and float64 legitimately disagree); no other class is allowed.
 Images: per-pixel means against the checker's independent render with a Welch z
 per pixel and channel.
 """
@@ -15,7 +16,8 @@ import pytest
 from spica_amd import configs, meshio
 from spica_amd.scene import Scene, diffuse
 from tests.conftest import GOLDEN
-from tests.helpers import classify_mismatches, furnace_room, ray_set, small_mixed_scene, wide_materials_scene, zstats
+from tests.helpers import (classify_mismatches, furnace_room, ray_set, small_mixed_scene, wide_materials_scene, worst_pixels,
+                           zstats)
 
 pytestmark = pytest.mark.gpu
 
@@ -264,8 +266,15 @@ def _render_cpu(oracle, scene, cam, rd):
 def _z_report(zs, name):
     a = np.abs(zs)
     f3 = float(np.mean(a > 3))
-    print(f"{name}: |z|>3 {f3:.4%}, |z|>4 {np.mean(a > 4):.4%}, max |z| {a.max():.2f}")
+    print(f"{name}: |z|>3 {f3:.4%}, |z|>4 {np.mean(a > 4):.4%}, max |z| {a.max():.2f}", flush=True)
     return f3, float(a.max())
+
+
+def _assert_parity(zs, name):
+    """The pixel criterion: at most 0.5% of pixel-channels with |z| > 3 and none with |z| > 6."""
+    f3, zmax = _z_report(zs, name)
+    assert f3 <= 0.005, f"{name}: {f3:.4%} of pixel-channels with |z| > 3"
+    assert zmax < 6.0, f"{name}: max |z| {zmax:.2f}"
 
 
 @pytest.mark.parametrize("depth", [0, 1, 4])
@@ -297,8 +306,7 @@ def test_cornell_means_within_3_sigma(gpu, oracle):
     sc, qc, cc = _render_cpu(oracle, w.scene, w.camera, w.render_desc(spp=512, seed=777))
     _dump("cornell_films", sg=sg, qg=qg, cg=cg, sc=sc, qc=qc, cc=cc)
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, "cornell")
-    assert f3 <= 0.005
+    _assert_parity(z, "cornell")
     assert abs(mg.mean() - mc.mean()) < 3e-3 * mc.mean()
 
 
@@ -310,8 +318,7 @@ def test_mixed_materials_within_3_sigma(gpu, oracle, sampler):
     sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 2048)
     sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, sampler=0, seed=99))
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, f"mixed sampler={sampler}")
-    assert f3 <= 0.01
+    _assert_parity(z, f"mixed sampler={sampler}")
 
 
 def test_envmap_within_3_sigma(gpu, oracle):
@@ -326,8 +333,49 @@ def test_envmap_within_3_sigma(gpu, oracle):
     sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 1024)
     sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, seed=41))
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, "envmap")
-    assert f3 <= 0.01
+    _assert_parity(z, "envmap")
+
+
+def test_envmap_rotated_within_3_sigma(gpu, oracle):
+    """A non-symmetric, translated environment transform (spica keeps toWorld transposed,
+    envmap.cc:19) over a map that varies along u and v with an off-axis sun: the GPU's lookup,
+    tables and sampling orientation agree with the checker's in every pixel."""
+    s, cam = small_mixed_scene(n_spheres=16)
+    h, w = 32, 64
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    sky = (0.5 + 1.5 * (1 - v))[:, None] * (0.6 + 0.8 * u)[None, :]
+    img = np.stack([sky, 0.8 * sky, 1.2 * sky], axis=-1).astype(np.float32)
+    img[5:7, 20:23] = 400.0
+    c, sn = np.cos(np.radians(30.0)), np.sin(np.radians(30.0))
+    rz = np.array([[c, -sn, 0, 0], [sn, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])
+    rx = np.array([[1, 0, 0, 0], [0, 0, -1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], float)
+    M = rz @ rx
+    M[:3, 3] = [3.0, -1.0, 2.0]
+    assert not np.allclose(M[:3, :3], M[:3, :3].T)
+    s.set_envmap(img, to_world=M, radius=30.0)
+    wl = configs.Workload("env-rot", s, cam, 1024, 8, 0, 5)
+    sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 1024)
+    sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, seed=42))
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    _assert_parity(z, "envmap rotated")
+
+
+def test_thin_lens_within_3_sigma(gpu, oracle):
+    """Cornell box through a thin lens (apertureRadius 0.04, focused on the front box,
+    perspective.cc:62-70): GPU vs the checker, whose lens is pinned by closed forms."""
+    from spica_amd.scene import Camera, look_at
+
+    w = configs.cornell_box(width=64, height=64)
+    cam = Camera(64, 64, look_at((0.5, 0.5, -1.37), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0)), 40.0,
+                 aperture_radius=0.04, focus_distance=1.6)
+    sg, qg, cg, _ = _render_gpu(gpu, w.scene, cam, w.render_desc(spp=512), 512)
+    sc, qc, cc = _render_cpu(oracle, w.scene, cam, w.render_desc(spp=512, seed=778))
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    _assert_parity(z, "thin lens")
+    # the lens blurs: the film differs from the pinhole render in the same seed
+    sp, _, cp, _ = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=512), 512)
+    assert not np.array_equal(sp, sg)
 
 
 @pytest.mark.parametrize("depth", [16, 3])
@@ -346,8 +394,7 @@ def test_direct_lighting_within_3_sigma(gpu, oracle, depth):
     sg, qg, cg, _ = _render_gpu(gpu, s, cam, wl.render_desc(), 1024)
     sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=1024, seed=43))
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, f"direct depth={depth}")
-    assert f3 <= 0.01
+    _assert_parity(z, f"direct depth={depth}")
     assert abs(mg.mean() - mc.mean()) < 5e-3 * mc.mean()
 
 
@@ -359,8 +406,7 @@ def test_direct_lighting_cornell_and_no_guide(gpu, oracle):
     sg, qg, cg, _ = _render_gpu(gpu, w.scene, w.camera, rd, 256)
     sc, qc, cc = _render_cpu(oracle, w.scene, w.camera, w.render_desc(spp=256, seed=5, integrator=1))
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, "direct cornell")
-    assert f3 <= 0.01
+    _assert_parity(z, "direct cornell")
     # direct lighting alone is darker than the full path integral
     sp, _, cp, _ = _render_gpu(gpu, w.scene, w.camera, w.render_desc(spp=64), 64)
     assert mg.mean() < 0.9 * (sp.sum() / cp.sum() / 3)
@@ -385,8 +431,10 @@ def test_wide_materials_within_3_sigma(gpu, oracle, integrator, sampler):
     # equal sample counts: with heavy tails a smaller count biases its sample mean low
     sc, qc, cc = _render_cpu(oracle, s, cam, wl.render_desc(spp=2048, sampler=0, seed=77))
     z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
-    f3, zmax = _z_report(z, f"wide integrator={integrator} sampler={sampler}")
-    assert f3 <= 0.01
+    _dump(f"wide_{integrator}_{sampler}", sg=sg, qg=qg, cg=cg, sc=sc, qc=qc, cc=cc)
+    for line in worst_pixels(z, sg, qg, cg[:, None], sc, qc, cc[:, None]):
+        print("  ", line)
+    _assert_parity(z, f"wide integrator={integrator} sampler={sampler}")
     # glass and the plastics' BSDF-only paths make the image heavy-tailed: the whole-image
     # mean is held to 4 standard errors of its own estimate rather than a fixed fraction
     vg = np.maximum(qg / cg[:, None] - mg ** 2, 0.0) / cg[:, None]
@@ -419,5 +467,29 @@ def test_relmse_kernel_matches_numpy(gpu):
     want = np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2))
     got = gr.relmse(ref, 1e-2)
     assert abs(got - want) <= 1e-5 * want
+    gr.close()
+    gs.close()
+
+
+def test_relmse_pixels_frees_its_buffers(gpu):
+    """spk_render_relmse_pixels uploads the pixel list and the reference per call; repeated
+    calls (bench.py checks every 32 samples) must leave device memory where it was."""
+    import torch
+
+    w = configs.cornell_box(width=512, height=512)
+    gs = gpu.GpuScene(w.scene)
+    gr = gpu.GpuRender(gs, w.camera, w.render_desc(spp=1))
+    gr.run(1)
+    n = 200_000
+    px = np.arange(n, dtype=np.int32)
+    ref = np.full((n, 3), 0.5)
+    first = gr.relmse_pixels(px, ref)
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    for _ in range(200):  # 5.6 MB a call: 1.1 GB if the buffers leaked
+        assert gr.relmse_pixels(px, ref) == first
+    free1, _ = torch.cuda.mem_get_info()
+    print(f"device memory free before {free0 / 2**20:.1f} MiB, after {free1 / 2**20:.1f} MiB")
+    assert free0 - free1 < 64 * 2**20
     gr.close()
     gs.close()

@@ -161,9 +161,10 @@ def test_guided_render_is_unbiased(gpu):
     (m0, v0, c0), (m1, v1, c1) = res
     se = np.sqrt(v0 / c0 + v1 / c1)
     z = np.where(se > 0, (m1 - m0) / np.maximum(se, 1e-30), 0.0)
-    f3, f4 = np.mean(np.abs(z) > 3), np.mean(np.abs(z) > 4)
-    print(f"guided vs plain cornell: |z|>3 {100 * f3:.3f}%, |z|>4 {100 * f4:.4f}%, means {m0.mean():.5f} {m1.mean():.5f}")
-    assert f3 < 0.01 and f4 < 0.001
+    f3, zmax = np.mean(np.abs(z) > 3), np.abs(z).max()
+    print(f"guided vs plain cornell: |z|>3 {100 * f3:.3f}%, max |z| {zmax:.2f}, means {m0.mean():.5f} {m1.mean():.5f}")
+    # the pixel criterion: at most 0.5% of pixel-channels beyond 3 sigma, none beyond 6
+    assert f3 <= 0.005 and zmax < 6.0
     assert abs(m1.mean() / m0.mean() - 1) < 0.01
     gs.close()
 
@@ -196,6 +197,38 @@ def test_guided_render_reduces_variance(gpu):
           f"ratio {out[False] / out[True]:.2f}")
     # about 1 training record per pixel and round: the gain grows with the frame (3x at 1280x720)
     assert out[False] / out[True] > 1.5
+    gs.close()
+
+
+def test_guided_training_spans_passes(gpu):
+    """A training round of 4 spp runs as one pass at a large queue, as 4 passes when the queue
+    holds one sample of the frame, and as 12 when a sample needs 3 chunks.  Each pass turns its
+    records' radiance into the radiance gathered after the vertex before the next pass reuses
+    the path slots, and records are kept by (pixel, depth, sample), so the three renders train
+    on the same records and give the same film bit for bit."""
+    from spica_amd import configs, render
+    wl = configs.c3_occluded_glossy(width=160, height=90)
+    npix = 160 * 90
+    gs = render.GpuScene(wl.scene)
+    films, steps, recs = [], [], []
+    for q in (1 << 22, npix, 5000):
+        gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=48), queue_size=q, block_size=64)
+        g = render.Guide(batch_size=1 << 14, train_spp=32, train_pass_spp=4, steps_per_pass=2)
+        gr.set_guide(g)
+        gr.run(48)
+        st = gr.stats()
+        assert st["train_passes"] == 8
+        films.append(gr.film())
+        steps.append(g.stats()["steps"])
+        recs.append(g.stats()["last_records"])
+        print(f"queue {q}: passes {st['passes']}, guided vertices {st['guided_vertices']}, "
+              f"records of the last round {recs[-1]}, film sum {films[-1][0].sum():.9g}", flush=True)
+        gr.close()
+        g.close()
+    assert steps[0] == steps[1] == steps[2] == 16 and recs[0] == recs[1] == recs[2] > 0
+    for f in films[1:]:
+        for a, b in zip(films[0], f):
+            assert np.array_equal(a, b)
     gs.close()
 
 

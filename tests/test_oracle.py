@@ -308,6 +308,182 @@ def test_envmap_pdf_integrates_and_matches_sampling(oracle):
     assert sun.mean() > 0.3
 
 
+def _light_uv(dirs_light):
+    """Spherical coordinates of light-space directions as spica's Envmap::Le takes them
+    (vect_math.h:46-54, envmap.cc:130-134): u = phi / 2pi with phi in [0, 2pi), v = theta / pi."""
+    phi = np.arctan2(dirs_light[:, 1], dirs_light[:, 0])
+    phi = np.where(phi < 0, phi + 2 * np.pi, phi)
+    theta = np.arccos(np.clip(dirs_light[:, 2], -1, 1))
+    return phi / (2 * np.pi), theta / np.pi
+
+
+def _linear_texels(x, n, a, b):
+    """Bilinear interpolation of texels a + b*i (i = 0..n-1, repeat wrap) at continuous texel
+    coordinate x = uv * n - 0.5, with the integer part truncated toward zero (mipmap.cc:81-90).
+    Interpolation of a linear sequence is exact: a + b*x, also on (-1, 0) where truncation
+    makes it extrapolate from texels 0 and 1; only [n-1, n - 1/2) mixes in the wrapped texel 0."""
+    x = np.asarray(x)[:, None]
+    lin = a + b * x
+    d = x - (n - 1)
+    wrap = (1 - d) * (a + b * (n - 1)) + d * a
+    return np.where(x >= n - 1, wrap, lin)
+
+
+def test_envmap_lookup_linear_in_u(oracle):
+    """Le of a map linear along u pins the u = phi/2pi convention (no mirroring), the half-texel
+    offset, bilinear (not nearest) filtering at level 0 and the truncating texel index: on the
+    first half texel a floor would interpolate towards the wrapped last texel instead."""
+    h, w = 8, 16
+    a = np.array([1.0, 2.0, 3.0])
+    b = np.array([0.5, 0.25, -0.125])
+    img = (a + b * np.arange(w)[:, None])[None].repeat(h, axis=0).astype(np.float32)
+    sc = oracle.scene(_env_scene(img))
+    rng = np.random.default_rng(11)
+    dirs = unit_vectors(rng, 20000)
+    le, _ = sc.env_eval(dirs)
+    u, _ = _light_uv(dirs)
+    want = _linear_texels(u * w - 0.5, w, a, b)
+    assert np.allclose(le, want, rtol=1e-9, atol=1e-12)
+    # the discriminating regions are populated: extrapolated first half texel, wrapped last one
+    assert np.sum(u * w - 0.5 < 0) > 200 and np.sum(u * w - 0.5 >= w - 1) > 200
+    # a mirrored u or nearest-texel lookup would differ by a texel slope or more
+    assert np.abs(_linear_texels((1 - u) * w - 0.5, w, a, b) - want).max() > 1.0
+
+
+def _rows_pdf(a, b, h):
+    """Sampling weight per row of a map whose rows are a + b*r: spica samples the level-1 mip
+    (2x2 box averages, rows a + b*(2j + 1/2)) looked up bilinearly at level-0 row centres
+    (envmap.cc:35-46, filter 1/max(w, h) selects level 1 of a power-of-two map), times
+    sin(theta).  Bilinear interpolation of those linear rows returns a + b*r again, except in
+    the last row, which mixes the last level-1 row (weight 3/4) with the wrapped first one
+    (1/4): a + b*(0.75 h - 1).  Returns gray(row value) * sin(theta_r)."""
+    c = np.arange(h, dtype=np.float64)
+    c[-1] = 0.75 * h - 1.0
+    vals = a[None, :] + b[None, :] * c[:, None]
+    g = 0.2126 * vals[:, 0] + 0.7152 * vals[:, 1] + 0.0722 * vals[:, 2]
+    return g * np.sin(np.pi * (np.arange(h) + 0.5) / h)
+
+
+def _rot(axis, deg):
+    c, s = math.cos(math.radians(deg)), math.sin(math.radians(deg))
+    m = np.eye(4)
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]
+    m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+    return m
+
+
+@pytest.mark.parametrize("to_world", ["identity", "rotated", "rotated+translated"])
+def test_envmap_linear_in_v_lookup_pdf_and_sampling(oracle, to_world):
+    """A map linear along v (theta).  Le follows the truncating bilinear rule along theta; the
+    density is proportional to the level-1 lookup times sin(theta) with closed-form rows
+    (_rows_pdf); sampled directions fill the rows in those proportions and uniformly in phi.
+    With a non-symmetric toWorld R, spica's light keeps R transposed (envmap.cc:19), so a world
+    direction wd is looked up at R wd, and samples leave along R^T of their map direction;
+    a translation in toWorld changes nothing for directions."""
+    h, w = 8, 16
+    a = np.array([4.0, 3.0, 2.0])
+    b = np.array([-0.25, 0.125, 0.5])
+    img = (a + b * np.arange(h)[:, None])[:, None, :].repeat(w, axis=1).astype(np.float32)
+    M = np.eye(4)
+    if to_world != "identity":
+        M = _rot(2, 30.0) @ _rot(0, 90.0)
+        assert not np.allclose(M[:3, :3], M[:3, :3].T)
+    if to_world == "rotated+translated":
+        M[:3, 3] = [5.0, -2.0, 1.0]
+    R = M[:3, :3]
+    sc = oracle.scene(_env_scene(img, to_world=M))
+    rng = np.random.default_rng(12)
+    dirs = unit_vectors(rng, 20000)
+    le, pdf = sc.env_eval(dirs)
+    light = dirs @ R.T  # R wd
+    _, v = _light_uv(light)
+    assert np.allclose(le, _linear_texels(v * h - 0.5, h, a, b), rtol=1e-9, atol=1e-12)
+    if to_world != "identity":
+        _, v_wrong = _light_uv(dirs @ R)  # an untransposed toWorld would look up R^T wd
+        assert np.abs(_linear_texels(v_wrong * h - 0.5, h, a, b) - le).max() > 0.5
+    g = _rows_pdf(a, b, h)
+    row = np.clip((v * h).astype(int), 0, h - 1)
+    st = np.sqrt(np.maximum(0, 1 - light[:, 2] ** 2))
+    want_pdf = g[row] / (g.sum() / h) / (2 * np.pi ** 2 * st)
+    assert np.allclose(pdf, want_pdf, rtol=1e-9)
+    # sampling: rows in proportion to g, phi uniform
+    n = 200000
+    sd, spdf, li = sc.env_sample(rng.random((n, 2)))
+    sl = sd @ R.T
+    su, sv = _light_uv(sl)
+    counts = np.bincount(np.clip((sv * h).astype(int), 0, h - 1), minlength=h)
+    exp = n * g / g.sum()
+    chi2 = float(((counts - exp) ** 2 / exp).sum())
+    assert chi2 < 30.0, (counts, exp)  # 7 dof: p ~ 1e-4
+    pc = np.bincount(np.clip((su * w).astype(int), 0, w - 1), minlength=w)
+    chi2u = float(((pc - n / w) ** 2 / (n / w)).sum())
+    assert chi2u < 45.0, pc  # 15 dof
+    # returned radiance and pdf are the evaluated ones
+    le2, pdf2 = sc.env_eval(sd)
+    assert np.allclose(li, le2, rtol=1e-6) and np.allclose(spdf, pdf2, rtol=1e-6)
+
+
+def test_envmap_one_hot_texel(oracle):
+    """A single bright texel: Le at its centre is its value (bilinear weights 1, 0, 0, 0), a
+    direction half a texel away in u sees half of it; nearly every sample lands in its 2x2
+    level-1 block, whose neighbourhood is all that the level-1 lookup can reach."""
+    h, w = 16, 32
+    img = np.zeros((h, w, 3), np.float32)
+    r0, c0 = 5, 20
+    img[r0, c0] = [100.0, 50.0, 25.0]
+    img += 1e-3
+    sc = oracle.scene(_env_scene(img))
+    th, ph = (r0 + 0.5) / h * np.pi, (c0 + 0.5) / w * 2 * np.pi
+    dctr = np.array([[np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]])
+    ph2 = (c0 + 1.0) / w * 2 * np.pi
+    dhalf = np.array([[np.sin(th) * np.cos(ph2), np.sin(th) * np.sin(ph2), np.cos(th)]])
+    le, _ = sc.env_eval(np.concatenate([dctr, dhalf]))
+    assert np.allclose(le[0], img[r0, c0], rtol=1e-6)
+    assert np.allclose(le[1], 0.5 * (img[r0, c0] + img[r0, c0 + 1]), rtol=1e-6)
+    sd, _, _ = sc.env_sample(np.random.default_rng(13).random((20000, 2)))
+    su, sv = _light_uv(sd)
+    ci, ri = (su * w).astype(int), (sv * h).astype(int)
+    near = (np.abs(ci - c0) <= 2) & (np.abs(ri - r0) <= 2)
+    assert near.mean() > 0.99
+
+
+def test_thin_lens_focus_and_disc(oracle):
+    """PerspectiveCamera::spawnRay with apertureRadius > 0 (perspective.cc:62-70): every lens
+    sample's ray passes through the point where the pixel's pinhole ray crosses the plane at
+    focusDistance along the view axis; origins lie on the lens disc of that radius, spanned by
+    the camera's x ("left") and y axes, filled uniformly (the concentric map preserves area,
+    sampling.cc:151-164), with the concentric map's known answers."""
+    from scipy import stats
+
+    M = look_at((1.0, 2.0, -3.0), (0.5, 1.0, 4.0), (0.1, 1.0, 0.0))
+    R, focus = 0.3, 7.5
+    thin = Camera(64, 48, M, 40.0, aperture_radius=R, focus_distance=focus)
+    pin = Camera(64, 48, M, 40.0)
+    eye, left, up, fwd = M[:3, 3], M[:3, 0], M[:3, 1], M[:3, 2]
+    rng = np.random.default_rng(14)
+    rr = []
+    for x, y, f0, f1 in [(0, 0, 0.5, 0.5), (63, 47, 0.9, 0.1), (20, 30, 0.3, 0.7)]:
+        o0, d0 = oracle.camera_ray(pin, x, y, [f0, f1, 0.5, 0.5])
+        pf = o0 + d0 * (focus / np.dot(d0, fwd))
+        assert abs(np.dot(pf - eye, fwd) - focus) < 1e-9
+        for l0, l1 in rng.random((400, 2)):
+            o, d = oracle.camera_ray(thin, x, y, [f0, f1, l0, l1])
+            off = o - eye
+            assert abs(np.dot(off, fwd)) < 1e-9 and np.linalg.norm(off) <= R * (1 + 1e-12)
+            rr.append((np.dot(off, left), np.dot(off, up)))
+            to_f = pf - o
+            assert np.linalg.norm(np.cross(to_f, d)) < 1e-9 * np.linalg.norm(to_f) and np.dot(to_f, d) > 0
+            assert abs(np.linalg.norm(d) - 1.0) < 1e-12
+    rr = np.array(rr) / R
+    assert stats.kstest((rr ** 2).sum(1), "uniform").pvalue > 1e-3
+    assert stats.kstest((np.arctan2(rr[:, 1], rr[:, 0]) + np.pi) / (2 * np.pi), "uniform").pvalue > 1e-3
+    # concentric map known answers: u = (0.75, 0.5) -> (r, theta) = (0.5, 0); u = (0.5, 0.9) -> (0.8, pi/2)
+    o, _ = oracle.camera_ray(thin, 10, 10, [0.5, 0.5, 0.75, 0.5])
+    assert np.allclose(o, eye + 0.5 * R * left, atol=1e-12)
+    o, _ = oracle.camera_ray(thin, 10, 10, [0.5, 0.5, 0.5, 0.9])
+    assert np.allclose(o, eye + 0.8 * R * up, atol=1e-12)
+
+
 def test_camera_closed_form(oracle):
     cam = Camera(65, 33, look_at((1, 2, 3), (1, 2, 10), (0, 1, 0)), 40.0)
     o, d = oracle.camera_ray(cam, 32, 16, [0.5, 0.5, 0.5, 0.5])

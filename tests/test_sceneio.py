@@ -116,6 +116,34 @@ def test_rgbe_known_answer_and_round_trip(tmp_path):
     assert np.all(np.abs(back - img) <= big / 128.0 + 1e-30)
 
 
+def test_rgbe_whole_file_known_answer(tmp_path):
+    """Every byte of a small file as Image::saveHdr writes it (image.cc:390-435): spica's five
+    header lines, "-Y h +X w", then per scanline 2, 2, w (big endian) and the R, G, B, E planes
+    in literal runs of at most 127, each pixel HDRPixel's (c * m * 256 / max, e + 128) with
+    max = m * 2^e (image.cc:72-88), worked out by hand below."""
+    img = np.zeros((2, 3, 3), dtype=np.float64)
+    img[0, 0] = (1.0, 0.5, 0.25)  # max 1 = 0.5 * 2^1: scale 128 -> (128, 64, 32, 129)
+    img[0, 1] = (3.0, 2.0, 1.0)   # max 3 = 0.75 * 2^2: scale 64 -> (192, 128, 64, 130)
+    img[0, 2] = (1e-33, 0, 0)     # below 1e-32: black (0, 0, 0, 0)
+    img[1, 0] = (2.0, 4.0, 8.0)   # max 8 = 0.5 * 2^4: scale 16 -> (32, 64, 128, 132)
+    img[1, 1] = (0.0, 0.0, 0.0)
+    img[1, 2] = (0.75, 0.0, 0.375)  # max 0.75 = 0.75 * 2^0: scale 256 -> (192, 0, 96, 128)
+    want = (b"#?RADIANCE\n# Made with 100% pure HDR Shop\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n"
+            b"-Y 2 +X 3\n"
+            + bytes([2, 2, 0, 3, 3, 128, 192, 0, 3, 64, 128, 0, 3, 32, 64, 0, 3, 129, 130, 0])
+            + bytes([2, 2, 0, 3, 3, 32, 0, 192, 3, 64, 0, 0, 3, 128, 0, 96, 3, 132, 0, 128]))
+    sceneio.write_rgbe(tmp_path / "k.hdr", img)
+    assert (tmp_path / "k.hdr").read_bytes() == want
+    # a scanline longer than 127 pixels is cut into literal runs of 127 and the rest
+    wide = np.full((1, 130, 3), 1.0)
+    sceneio.write_rgbe(tmp_path / "w.hdr", wide)
+    raw = (tmp_path / "w.hdr").read_bytes()
+    body = raw[raw.index(b"-Y 1 +X 130\n") + len(b"-Y 1 +X 130\n"):]
+    plane = bytes([127]) + bytes([128]) * 127 + bytes([3]) + bytes([128]) * 3
+    eplane = bytes([127]) + bytes([129]) * 127 + bytes([3]) + bytes([129]) * 3
+    assert body == bytes([2, 2, 0, 130]) + plane * 3 + eplane
+
+
 def test_rgbe_black_threshold(tmp_path):
     # HDRPixel (image.cc:72-88): a pixel whose largest channel is <= 1e-32 is written as zero
     img = np.array([[[1e-32, 0, 0], [2e-32, 0, 0]]], dtype=np.float64)
