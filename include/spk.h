@@ -81,6 +81,17 @@ int spk_scene_stats_get(const spk_scene* scene, spk_scene_stats* out);
 int spk_trace(spk_scene* scene, const float* org, const float* dir, const float* tmax, int64_t n, int32_t any_hit,
               int32_t* prim_out, float* t_out, void* stream);
 
+/* spk_trace without allocation.  No counterpart in spica: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as spk_trace, bit for bit; the
+ * traversal scratch (each persistent thread's stack overflow) is a fixed 32 MiB array per
+ * GPU, held from the library's load to the end of the process, instead of per-scene
+ * allocations.  A scene whose BVH needs more stack overflow per 128-thread block than the
+ * array holds is refused with SPK_ERR_INVALID; deep trees run a smaller grid.  All calls on
+ * one GPU share the array: calls that overlap on one GPU race. */
+int spk_trace_noalloc(spk_scene* scene, const float* org, const float* dir, const float* tmax, int64_t n,
+                      int32_t any_hit, int32_t* prim_out, float* t_out, void* stream);
+
 /* Tuning knobs that spica does not have (its CPU loop needs none). */
 typedef struct spk_render_opts {
     int64_t queue_size;     /* paths in flight per pass (0 = auto: 2^21) */

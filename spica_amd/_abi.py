@@ -217,6 +217,9 @@ SIGNATURES = {
     "spk_scene_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkSceneStats)]),
     "spk_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                  ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "spk_trace_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
     "spk_render_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkCameraDesc), ctypes.POINTER(SpkRenderDesc),
                                          ctypes.POINTER(SpkRenderOpts), ctypes.POINTER(ctypes.c_void_p)]),
     "spk_render_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -7,7 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspk.so")
 
-SOURCES = ["spk_kernels.hip", "spk_guide.hip", "spk_api.cpp", "spk_guide_api.cpp", "bvh_build.cpp", "lbvh.hip"]
+SOURCES = ["spk_kernels.hip", "spk_guide.hip", "spk_api.cpp", "spk_guide_api.cpp", "bvh_build.cpp", "lbvh.hip",
+           "spk_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -44,8 +44,10 @@ class GpuScene:
         _abi.check(self.lib.spk_scene_stats_get(self.handle, ctypes.byref(st)), "spk_scene_stats_get")
         return {k: getattr(st, k) for k, _ in st._fields_}
 
-    def trace(self, org, dir, tmax=None, any_hit=False, stream=None):
-        """org/dir: float32 [n, 3] device tensors; returns (prim int32 [n], t float32 [n] or None)."""
+    def trace(self, org, dir, tmax=None, any_hit=False, stream=None, noalloc=False):
+        """org/dir: float32 [n, 3] device tensors; returns (prim int32 [n], t float32 [n] or None).
+        noalloc: trace through spk_trace_noalloc, whose scratch is a fixed per-GPU array (no
+        allocation; calls overlapping on one GPU race on it)."""
         import torch
 
         assert org.is_cuda and dir.is_cuda and org.dtype == torch.float32 and dir.dtype == torch.float32
@@ -59,8 +61,9 @@ class GpuScene:
         prim = torch.empty(n, dtype=torch.int32, device=org.device)
         t = None if any_hit else torch.empty(n, dtype=torch.float32, device=org.device)
         st = torch.cuda.current_stream().cuda_stream if stream is None else stream
-        _abi.check(self.lib.spk_trace(self.handle, _ptr(org), _ptr(dir), _ptr(tmax), n, 1 if any_hit else 0, _ptr(prim),
-                                      _ptr(t), ctypes.c_void_p(st)), "spk_trace")
+        fn, name = (self.lib.spk_trace_noalloc, "spk_trace_noalloc") if noalloc else (self.lib.spk_trace, "spk_trace")
+        _abi.check(fn(self.handle, _ptr(org), _ptr(dir), _ptr(tmax), n, 1 if any_hit else 0, _ptr(prim), _ptr(t),
+                      ctypes.c_void_p(st)), name)
         return prim, t
 
     def close(self):
