@@ -135,12 +135,16 @@ def classify_mismatches(verts, org, dirv, p_gpu, t_gpu, p_cpu, t_cpu, rel_t=1e-6
     return out
 
 
-# Relative precision of a float32 film mean against a float64 one: per-sample radiance in
+# Precision of a float32 film mean against a float64 one.  Relative: per-sample radiance in
 # float32 carries a few ulp (1e-7) per bounce, the Kahan film sums keep the mean to ~1e-7.
-# A pixel whose samples are all equal (an emitter seen directly, a black background) has
-# zero sample variance on both sides, and its means differ only by this precision; the
-# floor keeps z finite there and is far below any statistical standard error.
+# Absolute: products of the estimator's 1e-12 regularisers (Fresnel, tan theta) leave
+# radiance of 1e-35 .. 1e-45 (float32 denormals) where float64 gives 1e-99; such a pixel is
+# zero in both.  A pixel whose samples are all equal (an emitter seen directly, black
+# background, underflow residue) has zero sample variance on both sides and its means differ
+# only by this precision; the floor keeps z finite there and is far below any statistical
+# standard error of the scenes' pixels (their means are 1e-3 and up).
 FILM_REL_PRECISION = 1e-5
+FILM_ABS_PRECISION = 1e-12
 
 
 def zstats(sum_a, sq_a, n_a, sum_b, sq_b, n_b):
@@ -150,7 +154,7 @@ def zstats(sum_a, sq_a, n_a, sum_b, sq_b, n_b):
     mb = sum_b / n_b
     va = np.maximum(sq_a / n_a - ma * ma, 0.0) * n_a / (n_a - 1)
     vb = np.maximum(sq_b / n_b - mb * mb, 0.0) * n_b / (n_b - 1)
-    floor = FILM_REL_PRECISION * np.maximum(np.abs(ma), np.abs(mb))
+    floor = FILM_REL_PRECISION * np.maximum(np.abs(ma), np.abs(mb)) + FILM_ABS_PRECISION
     se = np.sqrt(va / n_a + vb / n_b + floor * floor)
     z = np.where(se > 0, (ma - mb) / np.where(se > 0, se, 1.0), np.where(np.abs(ma - mb) > 0, np.inf, 0.0))
     return z, ma, mb
