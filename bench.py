@@ -17,6 +17,7 @@ spica's CPU path (the FP64 C restatement, the reference's estimator and BVH) tim
 on this host's cores on a bounded sample.
 """
 import argparse
+import copy
 import json
 import os
 import sys
@@ -30,6 +31,11 @@ HBM_PEAK_GBS = 8000.0
 B_RAY_IO = 32 + 16          # ray in (o, d, tmax, id) + hit out (t, u, v, prim)
 B_NODE = 64                 # one 4-wide BVH node: four 8-bit-quantised child boxes against the node's box + links
 B_TRI = 48                  # one triangle: v0 | id, e1, e2
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+# guide network 224-64-64-64-64-40: FLOPs per vertex of one forward pass (2 per multiply-add)
+GUIDE_FWD_FLOP = 2 * (224 * 64 + 3 * 64 * 64 + 64 * 40)
+# one training sample: forward, back-propagated deltas of layers 5..2, and all weight gradients
+GUIDE_TRAIN_FLOP = GUIDE_FWD_FLOP + 2 * (64 * 40 + 3 * 64 * 64) + GUIDE_FWD_FLOP
 
 
 def log(*a):
@@ -196,6 +202,114 @@ def time_to_relmse(gs, w, ref, guide_cfg, target, max_spp, queue, block, chunk=3
                 last_relmse=float(f"{e:.4g}"), trace=trace, guided_vertices=st["guided_vertices"])
 
 
+def guide_roofline(gs3, w3, queue, block, guide_cfg):
+    """The guide kernel (one-blob MLP inference on MFMA + mixture/BSDF sampling per vertex) and the
+    training step, against the bf16 MFMA peak: a short profiled guided render (64 training + 32
+    samples; HIP events around every guide launch) and timed training steps on one batch."""
+    import numpy as np
+
+    from spica_amd import render
+
+    gr = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=96), queue_size=queue, block_size=block, profile=True)
+    g = render.Guide(**guide_cfg)
+    gr.set_guide(g)
+    gr.run(96, sync=True)
+    st = gr.stats()
+    gr.close()
+    g.close()
+    launches = max(1, st["guide_launches"])
+    verts = st["guided_vertices"] / launches
+    avg_s = st["guide_ms"] / 1e3 / launches
+    achieved = GUIDE_FWD_FLOP * verts / avg_s / 1e12
+    # training: steps on one host batch of batch_size records; the first call includes the upload
+    b = guide_cfg["batch_size"]
+    rng = np.random.default_rng(5)
+    x = rng.random((b, 7)).astype(np.float32)
+    d = rng.normal(size=(b, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    wgt = rng.random(b).astype(np.float32)
+    g = render.Guide(**guide_cfg)
+    g.train(x, d, wgt, steps=1)
+    t0 = time.perf_counter()
+    g.train(x, d, wgt, steps=1)
+    t1 = time.perf_counter()
+    g.train(x, d, wgt, steps=21)
+    t2 = time.perf_counter()
+    g.close()
+    step_s = ((t2 - t1) - (t1 - t0)) / 20
+    train_tflops = GUIDE_TRAIN_FLOP * b / step_s / 1e12
+    return dict(bound="mfma", kernel="k_guide (inference + per-vertex mixture/BSDF sampling)",
+                flop_per_vertex=GUIDE_FWD_FLOP, vertices_per_launch=round(verts, 1),
+                avg_launch_us=round(avg_s * 1e6, 2), achieved=round(achieved, 2), peak=BF16_PEAK_TFLOPS,
+                unit="TFLOP/s", frac=round(achieved / BF16_PEAK_TFLOPS, 4),
+                training=dict(flop_per_sample=GUIDE_TRAIN_FLOP, batch=b, step_us=round(step_s * 1e6, 1),
+                              achieved=round(train_tflops, 2), frac=round(train_tflops / BF16_PEAK_TFLOPS, 4),
+                              kernels="k_guide_fb + k_guide_dw + k_guide_adam"))
+
+
+def c3_leg(args, c3_ref):
+    """C3 time to relMSE 1e-3, plain and guided, over render seeds 7..10 (the Sobol seed; the
+    scene is seed 7's), with the ratio's decomposition: cost per sample (after training) and the
+    relMSE at 1024 spp (the error at equal samples)."""
+    from spica_amd import configs, render
+
+    w3 = configs.c3_occluded_glossy()
+    gs3 = render.GpuScene(w3.scene)
+    guide_cfg = dict(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8, learning_rate=1e-2,
+                     guide_fraction=0.5)
+    warm = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=16), queue_size=args.queue, block_size=args.block)
+    warm.run(16)
+    warm.close()
+    runs = []
+    for seed in args.c3_seeds:
+        ws = copy.copy(w3)  # the same scene and camera, another Sobol seed
+        ws.seed = seed
+        log(f"bench: C3 time to relMSE 1e-3, render seed {seed}, plain")
+        plain = time_to_relmse(gs3, ws, c3_ref, None, 1e-3, 16384, args.queue, args.block)
+        log(f"bench: plain {plain}")
+        log(f"bench: C3 time to relMSE 1e-3, render seed {seed}, guided")
+        guided = time_to_relmse(gs3, ws, c3_ref, guide_cfg, 1e-3, 16384, args.queue, args.block)
+        log(f"bench: guided {guided}")
+        ratio = None
+        if plain["seconds"] and guided["seconds"]:
+            ratio = round(guided["seconds"] / plain["seconds"], 4)
+        runs.append(dict(seed=seed, plain=plain, guided=guided, guided_over_plain=ratio))
+    roof = guide_roofline(gs3, w3, args.queue, args.block, guide_cfg)
+    log(f"bench: guide roofline {roof}")
+    gs3.close()
+
+    def at(trace, spp):
+        for k, t, e in trace:
+            if k == spp:
+                return t, e
+        return None, None
+
+    cost, var = [], []
+    for r in runs:
+        (tp, ep), (tg, eg) = at(r["plain"]["trace"], 1024), at(r["guided"]["trace"], 1024)
+        (tp64, _), (tg64, _) = at(r["plain"]["trace"], 64), at(r["guided"]["trace"], 64)
+        if None not in (tp, ep, tg, eg, tp64, tg64):
+            # per-sample cost after the 64 training samples, and the error at equal samples
+            cost.append(((tg - tg64) / 960) / ((tp - tp64) / 960))
+            var.append(eg / ep)
+    ratios = [r["guided_over_plain"] for r in runs if r["guided_over_plain"]]
+    mean = lambda v: round(sum(v) / len(v), 4) if v else None  # noqa: E731
+    first = runs[0]
+    return dict(workload="C3 occluded glossy: seed-7 sphere field (60% GGX alpha 0.1-0.3), light inside a box open "
+                         "towards the ceiling, 1280x720, maxDepth 16, Sobol",
+                target=1e-3, eps=1e-2,
+                reference=f"{c3_ref[2]}-spp float64 CPU render (spica's estimator), {len(c3_ref[0])} stratified pixels",
+                plain=first["plain"], guided=first["guided"], guided_over_plain=first["guided_over_plain"],
+                seeds=[dict(seed=r["seed"], plain_s=r["plain"]["seconds"], guided_s=r["guided"]["seconds"],
+                            guided_over_plain=r["guided_over_plain"]) for r in runs],
+                guided_over_plain_mean=mean(ratios),
+                decomposition=dict(cost_per_sample_guided_over_plain=mean(cost),
+                                   relmse_at_1024spp_guided_over_plain=mean(var),
+                                   note="guided/plain time ~ cost ratio x error ratio at equal samples"),
+                guide=dict(guide_cfg, network="one-blob(7x32) -> 64x4 ReLU -> 8 vMF lobes, bf16 MFMA, fp32 accum"),
+                guide_roofline=roof)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +325,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy-scene (C4) leg")
     ap.add_argument("--heavy-spp", type=int, default=256, help="samples per pixel of the C4 relMSE renders")
+    ap.add_argument("--c3-seeds", type=int, nargs="+", default=[7, 8, 9, 10],
+                    help="render seeds of the C3 time-to-relMSE leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -260,7 +376,12 @@ def main():
     v_node = cst["count_nodes"] / max(1, cst["count_rays"])
     v_tri = cst["count_tris"] / max(1, cst["count_rays"])
     b_ray = B_RAY_IO + B_NODE * v_node + B_TRI * v_tri
-    log(f"bench: V_node {v_node:.2f}  V_tri {v_tri:.2f}  B_ray {b_ray:.0f} B")
+    # occlusion rays: ray in (origin | path, direction | tmax) + contribution in, and the any-hit visits
+    v_node_s = cst["count_nodes_shadow"] / max(1, cst["count_rays_shadow"])
+    v_tri_s = cst["count_tris_shadow"] / max(1, cst["count_rays_shadow"])
+    b_shadow = 48 + 16 + B_NODE * v_node_s + B_TRI * v_tri_s
+    log(f"bench: V_node {v_node:.2f}  V_tri {v_tri:.2f}  B_ray {b_ray:.0f} B; occlusion V_node {v_node_s:.2f} "
+        f"V_tri {v_tri_s:.2f} B {b_shadow:.0f} B")
 
     gr = render.GpuRender(gs, w.camera, w.render_desc(spp=args.warmup + args.steps), queue_size=args.queue,
                           block_size=args.block, profile=True)
@@ -279,6 +400,27 @@ def main():
     ext_launches = st1["extend_launches"] - st0["extend_launches"]
     ext_rays = st1["extend_rays_timed"] - st0["extend_rays_timed"]
     value = rays / dt / 1e6
+    # shading: per hit the queue entry (48 B) and hit (16 B) in, the triangle's shading record
+    # (normal | material 16 B, ids 8 B), the path radiance read and written (32 B); per appended
+    # shadow, MIS or continuation ray 48 B out.  Materials and lights stay in cache.
+    sh_ms = st1["shade_ms"] - st0["shade_ms"]
+    sh_hits = ext_rays
+    sh_bytes = 120 * sh_hits + 48 * ((st1["rays_shadow"] - st0["rays_shadow"]) + (st1["rays_mis"] - st0["rays_mis"]) +
+                                     (st1["rays_extend"] - st0["rays_extend"]) - paths)
+    so_ms = st1["shadow_ms"] - st0["shadow_ms"]
+    so_rays = st1["shadow_rays_timed"] - st0["shadow_rays_timed"]
+    other_roofs = {
+        "k_shade": dict(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, bytes_per_hit=round(sh_bytes / max(1, sh_hits), 1),
+                        avg_launch_us=round(sh_ms * 1e3 / max(1, st1["shade_launches"] - st0["shade_launches"]), 2),
+                        achieved=round(sh_bytes / (sh_ms / 1e3) / 1e9, 2),
+                        frac=round(sh_bytes / (sh_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)),
+        "k_shadow": dict(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, b_ray=round(b_shadow, 1),
+                         v_node=round(v_node_s, 3), v_tri=round(v_tri_s, 3),
+                         avg_launch_us=round(so_ms * 1e3 / max(1, st1["shadow_launches"] - st0["shadow_launches"]), 2),
+                         achieved=round(b_shadow * so_rays / (so_ms / 1e3) / 1e9, 2),
+                         frac=round(b_shadow * so_rays / (so_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)),
+    }
+    log(f"bench: other kernels {other_roofs}")
     avg_launch_s = ext_ms / 1e3 / max(1, ext_launches)
     bytes_per_launch = b_ray * ext_rays / max(1, ext_launches)
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -297,29 +439,7 @@ def main():
     ttr = None
     c3_ref = load_reference("c3")
     if not args.no_guide and c3_ref is not None:
-        w3 = configs.c3_occluded_glossy()
-        gs3 = render.GpuScene(w3.scene)
-        guide_cfg = dict(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=2, learning_rate=1e-2,
-                         guide_fraction=0.5)
-        warm = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=16), queue_size=args.queue, block_size=args.block)
-        warm.run(16)
-        warm.close()
-        log("bench: C3 time to relMSE 1e-3, plain")
-        plain = time_to_relmse(gs3, w3, c3_ref, None, 1e-3, 16384, args.queue, args.block)
-        log(f"bench: plain {plain}")
-        log("bench: C3 time to relMSE 1e-3, guided")
-        guided = time_to_relmse(gs3, w3, c3_ref, guide_cfg, 1e-3, 16384, args.queue, args.block)
-        log(f"bench: guided {guided}")
-        ratio = None
-        if plain["seconds"] and guided["seconds"]:
-            ratio = round(guided["seconds"] / plain["seconds"], 4)
-        ttr = dict(workload="C3 occluded glossy: seed-7 sphere field (60% GGX alpha 0.1-0.3), light inside a box open "
-                            "towards the ceiling, 1280x720, maxDepth 16, Sobol seed 7",
-                   target=1e-3, eps=1e-2,
-                   reference=f"{c3_ref[2]}-spp float64 CPU render (spica's estimator), {len(c3_ref[0])} stratified pixels",
-                   plain=plain, guided=guided, guided_over_plain=ratio,
-                   guide=dict(guide_cfg, network="one-blob(7x32) -> 64x4 ReLU -> 8 vMF lobes, bf16 MFMA, fp32 accum"))
-        gs3.close()
+        ttr = c3_leg(args, c3_ref)
 
     batch = None
     if not args.no_batch:
@@ -434,26 +554,41 @@ def main():
                                max_depth=int(stl["max_depth"]), batch_seconds=round(ldt, 3),
                                mrays_per_s=round(lrays / ldt / 1e6, 3)))
         c4_ref = load_reference("c4")
-        if c4_ref is not None:
-            pix, refm, rspp = c4_ref
-            res = {}
-            for name, guided in (("plain", False), ("guided", True)):
-                v = views[0]
-                gr = render.GpuRender(gs4, v.camera, v.render_desc(spp=args.heavy_spp), queue_size=args.queue,
-                                      block_size=args.block)
-                g = None
-                if guided:
-                    g = render.Guide(train_spp=64, train_pass_spp=4, steps_per_pass=2)
-                    gr.set_guide(g)
-                t0 = time.perf_counter()
-                gr.run(args.heavy_spp, sync=True)
-                vdt = time.perf_counter() - t0
-                res[name] = dict(spp=args.heavy_spp, seconds=round(vdt, 3),
-                                 relmse=float(f"{gr.relmse_pixels(pix, refm):.5g}"))
-                gr.close()
-                if g is not None:
-                    g.close()
-            heavy["view0"] = dict(res, reference=f"{rspp}-spp float64 CPU render, {len(pix)} stratified pixels of view 0")
+        # the 8 views as one batch, plain and with one guide for all views (trained on the first
+        # 64 samples of every view, rounds of 4 across the views), to heavy_spp samples each
+        res = {}
+        for name, guided in (("plain", False), ("guided", True)):
+            items = [(gs4, v.camera, args.heavy_spp, v.seed) for v in views]
+            bh = render.Batch(items, max_depth=views[0].max_depth, sampler=views[0].sampler, queue_size=args.queue,
+                              block_size=args.block, graph=1)
+            g = None
+            if guided:
+                g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8)
+                bh.set_guide(g)
+            st0 = bh.stats()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bh.run(sync=True)
+            vdt = time.perf_counter() - t0
+            st1 = bh.stats()
+            hrays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
+            res[name] = dict(views=len(items), spp=args.heavy_spp, seconds=round(vdt, 3),
+                             mrays_per_s=round(hrays / vdt / 1e6, 3),
+                             mpaths_per_s=round((st1["paths"] - st0["paths"]) / vdt / 1e6, 3))
+            if c4_ref is not None:
+                import numpy as np
+
+                pix, refm, _ = c4_ref
+                sm, _, cn = bh.film(0)
+                m = (sm / np.maximum(cn, 1)[..., None]).reshape(-1, 3)[pix]
+                res[name]["relmse_view0"] = float(f"{float(((m - refm) ** 2 / (refm ** 2 + 1e-2)).mean()):.5g}")
+            if g is not None:
+                res[name]["guide_steps"] = int(g.stats()["steps"])
+                g.close()
+            bh.close()
+            log(f"bench: C4 8-view batch {name}: {res[name]}")
+        heavy["views_batch"] = dict(res, reference=None if c4_ref is None else
+                                    f"{c4_ref[2]}-spp float64 CPU render, {len(c4_ref[0])} stratified pixels of view 0")
         log(f"bench: heavy {heavy}")
         gs4.close()
 
@@ -494,6 +629,7 @@ def main():
             "avg_launch_us": round(avg_launch_s * 1e6, 2),
             "rays_per_launch": round(ext_rays / max(1, ext_launches), 1),
         },
+        "roofline_other": other_roofs,
         "relmse": c1_relmse,
         "time_to_relmse": ttr,
         "batch": batch,

@@ -153,6 +153,13 @@ typedef struct spk_render_stats {
     int64_t count_rays;     /* count_visits = 1: closest-hit rays those counts cover */
     int64_t guided_vertices;/* continuations sampled by the learned proposal (guided renders) */
     int64_t train_passes;   /* sample passes that recorded training data (guided renders) */
+    double guide_ms;        /* profile = 1, guided renders: summed guide-kernel (inference + sampling) time */
+    int64_t guide_launches;
+    int64_t shade_launches; /* profile = 1: shading launches timed in shade_ms */
+    int64_t shadow_rays_timed; /* profile = 1: occlusion rays of the timed shadow launches */
+    int64_t count_nodes_shadow; /* count_visits = 1: node fetches of the occlusion stage */
+    int64_t count_tris_shadow;
+    int64_t count_rays_shadow;
 } spk_render_stats;
 int spk_render_stats_get(spk_render* render, spk_render_stats* out);
 
@@ -227,7 +234,7 @@ int spk_render_set_guide(spk_render* render, spk_guide* guide);
  * sample and dimension, and every pixel accumulates in sample order).  Replaces running
  * spica's PathIntegrator::render once per scene file (sources/integrators/path/path.cc,
  * sources/core/integrator.cc:62-107).  Limits: W * H <= 2^22 per image, up to 1024
- * images, no guide (a guide serves single renders; views of one scene can share it). */
+ * images; a guide serves a batch whose images render one scene (spk_batch_set_guide). */
 typedef struct spk_batch spk_batch;
 
 typedef struct spk_batch_image {
@@ -248,6 +255,13 @@ typedef struct spk_batch_desc {
 
 int spk_batch_create(const spk_batch_desc* desc, const spk_render_opts* opts, spk_batch** out);
 int spk_batch_destroy(spk_batch* batch);
+/* Attach a learned-proposal guide to a batch whose images all render one scene (views of it),
+ * or detach it (NULL).  The first train_spp samples of every image then run first, as training
+ * rounds of train_pass_spp samples taken round by round across the images, each round followed
+ * by the guide's Adam steps; the rest of the samples use the trained network.  One network for
+ * all the views, trained online as spk_render_set_guide does for one view.  A guided batch
+ * launches its passes one by one (no graph replay). */
+int spk_batch_set_guide(spk_batch* batch, spk_guide* guide);
 /* Renders every image to its sample count (films are cleared first). */
 int spk_batch_run(spk_batch* batch, int32_t sync);
 int spk_batch_film(spk_batch* batch, int32_t image, double* sum_rgb, double* sumsq_rgb, double* count);

@@ -147,6 +147,13 @@ class SpkRenderStats(ctypes.Structure):
         ("count_rays", ctypes.c_int64),
         ("guided_vertices", ctypes.c_int64),
         ("train_passes", ctypes.c_int64),
+        ("guide_ms", ctypes.c_double),
+        ("guide_launches", ctypes.c_int64),
+        ("shade_launches", ctypes.c_int64),
+        ("shadow_rays_timed", ctypes.c_int64),
+        ("count_nodes_shadow", ctypes.c_int64),
+        ("count_tris_shadow", ctypes.c_int64),
+        ("count_rays_shadow", ctypes.c_int64),
     ]
 
 
@@ -248,6 +255,7 @@ SIGNATURES = {
     "spk_batch_create": (ctypes.c_int, [ctypes.POINTER(SpkBatchDesc), ctypes.POINTER(SpkRenderOpts),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "spk_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "spk_batch_set_guide": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "spk_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "spk_batch_film": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_d_p, c_d_p, c_d_p]),
     "spk_batch_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpkBatchInfo)]),

@@ -19,7 +19,7 @@
 // as the SAH tree (only exact-tie winners may differ); it is cheaper to build and somewhat
 // slower to traverse.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -249,13 +249,13 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
                        B.idx.as<uint32_t>());
     SPK_CHECK_HIP(hipGetLastError());
     size_t tbytes = 0;
-    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
-                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 63,
-                                                     (hipStream_t)0));
+    SPK_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
+                                            B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), (uint32_t)N, 0u, 63u,
+                                            (hipStream_t)0));
     if ((rc = B.tmp.alloc(std::max<size_t>(tbytes, 16)))) return rc;
-    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(B.tmp.p, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
-                                                     B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), N, 0, 63,
-                                                     (hipStream_t)0));
+    SPK_CHECK_HIP(rocprim::radix_sort_pairs(B.tmp.p, tbytes, B.codes.as<uint64_t>(), B.codes_s.as<uint64_t>(),
+                                            B.idx.as<uint32_t>(), B.idx_s.as<uint32_t>(), (uint32_t)N, 0u, 63u,
+                                            (hipStream_t)0));
     hipLaunchKernelGGL(k_hierarchy, dim3(blocks), dim3(256), 0, 0, B.codes_s.as<uint64_t>(), N, B.child.as<int2>(),
                        B.range.as<int2>(), B.parent.as<int>());
     SPK_CHECK_HIP(hipGetLastError());

@@ -262,6 +262,10 @@ struct spk_render {
     int64_t samples_done = 0;
     double extend_ms = 0.0;
     int64_t extend_launches = 0;
+    double guide_ms = 0.0;          // profile = 1: summed guide-kernel time
+    int64_t guide_launches = 0;
+    double shade_ms = 0.0, shadow_ms = 0.0;
+    int64_t shade_launches = 0, shadow_launches = 0, shadow_rays_timed = 0;
     int64_t extend_rays_timed = 0;
     std::vector<hipEvent_t> events;
     hipGraphExec_t pass_exec = nullptr;  // use_graph: one captured plain pass, replayed
@@ -895,6 +899,31 @@ GuideNetDev guide_net(spk_render* r, bool record) {
     return n;
 }
 
+// The Adam steps after a training round: the round's records sorted into a fixed order, then
+// steps_per_pass steps on batches drawn from them.
+int guide_train_round(spk_render* r, GuideNetDev net) {
+    spk_guide* g = r->guide;
+    float4* in[5];
+    float4* out[5];
+    for (int q = 0; q < 5; q++) {
+        in[q] = r->rec[q].as<float4>();
+        out[q] = r->rec_s[q].as<float4>();
+    }
+    int rc;
+    if ((rc = sort_guide_records(in, out, r->rec_cap, g->state.as<uint32_t>(), r->rec_keys.as<uint64_t>(),
+                                 r->rec_idx.as<uint32_t>(), r->rec_tmp.p, r->rec_tmp.bytes, r->stream)))
+        return rc;
+    for (int q = 0; q < 5; q++) net.rec[q] = out[q];
+    const TrainBufs tb = g->train_bufs(g->bcap);
+    for (int st = 0; st < g->opts.steps_per_pass; st++) {
+        uint32_t h0 = g->opts.seed, h1 = (uint32_t)g->steps, h2 = 0x6a09e667u, h3 = 0xbb67ae85u;
+        pcg4d(h0, h1, h2, h3);
+        if ((rc = launch_guide_step(g->bufs(), tb, &net, g->adam(), h0, 0u, true, nullptr, r->stream))) return rc;
+        g->steps++;
+    }
+    return SPK_OK;
+}
+
 // Samples [first, first + n_spp) as passes of the compiled pipeline.
 int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* net) {
     ConfigDev& c = r->cfg;
@@ -904,13 +933,16 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
     int rc;
     if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
     int64_t passes = c.chunks > 1 ? (int64_t)n_spp * c.chunks : ((int64_t)n_spp + c.kmax - 1) / c.kmax;
-    int nev = 2 * (c.max_depth + 1);
-    int64_t ext_before = 0;
+    // per depth: start/end of the closest-hit, guide (guided renders), shade and shadow launches
+    const int nd = c.max_depth + 1;
+    int nev = 8 * nd;
+    int64_t ext_before = 0, shadow_before = 0;
     if (r->opts.profile) {
         PassState ps;
         SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
         SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
         ext_before = (int64_t)ps.total_extend;
+        shadow_before = (int64_t)ps.total_shadow;
         size_t need = (size_t)passes * nev;
         while (r->events.size() < need) {
             hipEvent_t e;
@@ -946,12 +978,29 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
         PassState ps;
         SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
         r->extend_rays_timed += (int64_t)ps.total_extend - ext_before;
+        r->shadow_rays_timed += (int64_t)ps.total_shadow - shadow_before;
         for (int64_t p = 0; p < passes; p++)
             for (int d = 0; d <= c.max_depth; d++) {
                 float ms = 0.0f;
                 SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[(size_t)p * nev + 2 * d], r->events[(size_t)p * nev + 2 * d + 1]));
                 r->extend_ms += ms;
                 r->extend_launches++;
+                if (net && d < c.max_depth) {
+                    const size_t g = (size_t)p * nev + 2 * (nd + d);
+                    SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[g], r->events[g + 1]));
+                    r->guide_ms += ms;
+                    r->guide_launches++;
+                }
+                const size_t sh = (size_t)p * nev + 4 * nd + 2 * d;
+                SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[sh], r->events[sh + 1]));
+                r->shade_ms += ms;
+                r->shade_launches++;
+                if (d < c.max_depth) {
+                    const size_t so = (size_t)p * nev + 6 * nd + 2 * d;
+                    SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[so], r->events[so + 1]));
+                    r->shadow_ms += ms;
+                    r->shadow_launches++;
+                }
             }
     }
     return SPK_OK;
@@ -995,25 +1044,7 @@ int spk_render_run(spk_render* r, int32_t n_spp, int32_t sync) {
             net.rec_keep = 1.0f / (float)k;
             if ((rc = run_passes(r, first, k, &net))) return rc;
             if (train) {
-                float4* in[5];
-                float4* out[5];
-                for (int q = 0; q < 5; q++) {
-                    in[q] = r->rec[q].as<float4>();
-                    out[q] = r->rec_s[q].as<float4>();
-                }
-                if ((rc = sort_guide_records(in, out, r->rec_cap, g->state.as<uint32_t>(), r->rec_keys.as<uint64_t>(),
-                                             r->rec_idx.as<uint32_t>(), r->rec_tmp.p, r->rec_tmp.bytes, r->stream)))
-                    return rc;
-                for (int q = 0; q < 5; q++) net.rec[q] = out[q];
-                const TrainBufs tb = g->train_bufs(g->bcap);
-                for (int st = 0; st < g->opts.steps_per_pass; st++) {
-                    uint32_t h0 = g->opts.seed, h1 = (uint32_t)g->steps, h2 = 0x6a09e667u, h3 = 0xbb67ae85u;
-                    pcg4d(h0, h1, h2, h3);
-                    if ((rc = launch_guide_step(g->bufs(), tb, &net, g->adam(), h0, 0u, true,
-                                                nullptr, r->stream)))
-                        return rc;
-                    g->steps++;
-                }
+                if ((rc = guide_train_round(r, net))) return rc;
                 r->guide_trained += k;
                 r->train_passes++;
             }
@@ -1051,8 +1082,10 @@ int spk_render_set_guide(spk_render* r, spk_guide* g) {
     r->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)r->cfg.npix * 4u, uint64_t(1) << 24);
     for (int k = 0; k < 5; k++)
         if ((rc = r->rec[k].alloc((size_t)r->rec_cap * 16)) || (rc = r->rec_s[k].alloc((size_t)r->rec_cap * 16))) return rc;
+    size_t sort_bytes = 0;
+    if ((rc = guide_record_sort_bytes(r->rec_cap, &sort_bytes))) return rc;
     if ((rc = r->rec_keys.alloc((size_t)r->rec_cap * 16)) || (rc = r->rec_idx.alloc((size_t)r->rec_cap * 8)) ||
-        (rc = r->rec_tmp.alloc(std::max<size_t>(guide_record_sort_bytes(r->rec_cap), 16))))
+        (rc = r->rec_tmp.alloc(std::max<size_t>(sort_bytes, 16))))
         return rc;
     r->bufs.qg[0] = r->qg[0].as<float4>();
     r->bufs.qg[1] = r->qg[1].as<float4>();
@@ -1175,6 +1208,16 @@ int spk_render_stats_get(spk_render* r, spk_render_stats* out) {
     out->count_tris = (int64_t)ps.count_tris;
     out->count_rays = (int64_t)ps.count_rays;
     out->guided_vertices = (int64_t)ps.total_guided;
+    out->guide_ms = r->guide_ms;
+    out->guide_launches = r->guide_launches;
+    out->shade_ms = r->shade_ms;
+    out->shade_launches = r->shade_launches;
+    out->shadow_ms = r->shadow_ms;
+    out->shadow_launches = r->shadow_launches;
+    out->shadow_rays_timed = r->shadow_rays_timed;
+    out->count_nodes_shadow = (int64_t)ps.count_nodes_s;
+    out->count_tris_shadow = (int64_t)ps.count_tris_s;
+    out->count_rays_shadow = (int64_t)ps.count_rays_s;
     out->train_passes = r->train_passes;
     return SPK_OK;
 }
@@ -1222,6 +1265,9 @@ struct spk_batch {
     int64_t film_pixels = 0;
     int graph = 0;
     hipGraphExec_t exec = nullptr;    // one pass (graph = 1) or the whole batch (graph = 2)
+    // learned proposal (one network for every image of one scene)
+    std::vector<int32_t> pass_round;  // training round of each pass, -1 after training
+    std::vector<uint32_t> spp;        // samples per image
     ~spk_batch() {
         if (r) (void)hipSetDevice(r->device);
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1249,18 +1295,57 @@ void batch_chunks(uint32_t img, uint32_t npix, uint32_t spp, uint32_t cap, std::
     }
 }
 
+// Guided batches: the first `train_spp` samples of every image as training rounds of
+// `round_spp` samples, round-major (round r holds samples [r round_spp, (r + 1) round_spp) of
+// every image, so the network learns from every view), then the remaining samples; rounds[c]
+// is chunk c's round or -1.
+void batch_chunks_guided(const std::vector<uint32_t>& npix, const std::vector<uint32_t>& spp, uint32_t cap,
+                         uint32_t train_spp, uint32_t round_spp, std::vector<ChunkDev>& out,
+                         std::vector<int32_t>& rounds) {
+    uint32_t tmax = 0;
+    for (size_t i = 0; i < spp.size(); i++) tmax = std::max(tmax, std::min(train_spp, spp[i]));
+    const uint32_t nr = (tmax + round_spp - 1) / round_spp;
+    for (uint32_t rd = 0; rd < nr; rd++)
+        for (size_t i = 0; i < spp.size(); i++) {
+            const uint32_t s0 = rd * round_spp, s1 = std::min({(rd + 1) * round_spp, train_spp, spp[i]});
+            if (s0 >= s1) continue;
+            std::vector<ChunkDev> part;
+            batch_chunks((uint32_t)i, npix[i], s1 - s0, cap, part);
+            for (ChunkDev c : part) {
+                c.sample_base += s0;
+                out.push_back(c);
+                rounds.push_back((int32_t)rd);
+            }
+        }
+    for (size_t i = 0; i < spp.size(); i++) {
+        const uint32_t s0 = std::min(train_spp, spp[i]);
+        if (s0 >= spp[i]) continue;
+        std::vector<ChunkDev> part;
+        batch_chunks((uint32_t)i, npix[i], spp[i] - s0, cap, part);
+        for (ChunkDev c : part) {
+            c.sample_base += s0;
+            out.push_back(c);
+            rounds.push_back(-1);
+        }
+    }
+}
+
 // Packs chunks in order into passes of at most `cap` paths, at most one chunk per image
-// per pass (two chunks of one image in a pass could touch the same film pixel).
-void batch_passes(std::vector<ChunkDev>& ch, uint32_t cap, std::vector<PassDev>& out) {
+// per pass (two chunks of one image in a pass could touch the same film pixel), and, when
+// `rounds` is given, chunks of one training round only.
+void batch_passes(std::vector<ChunkDev>& ch, uint32_t cap, std::vector<PassDev>& out,
+                  const std::vector<int32_t>* rounds = nullptr, std::vector<int32_t>* pass_round = nullptr) {
     size_t i = 0;
     while (i < ch.size()) {
         PassDev p = {(uint32_t)i, (uint32_t)i, 0u, 0u};
         std::vector<uint32_t> imgs;
+        const int32_t rd = rounds ? (*rounds)[i] : -1;
         while (i < ch.size()) {
             ChunkDev& c = ch[i];
             const uint32_t paths = c.n_pix * c.k;
             if (p.n_paths + paths > cap) break;
             if (std::find(imgs.begin(), imgs.end(), c.img) != imgs.end()) break;
+            if (rounds && (*rounds)[i] != rd) break;
             c.path_off = p.n_paths;
             c.pix_off = p.n_pixels;
             p.n_paths += paths;
@@ -1270,6 +1355,7 @@ void batch_passes(std::vector<ChunkDev>& ch, uint32_t cap, std::vector<PassDev>&
         }
         p.chunk_end = (uint32_t)i;
         out.push_back(p);
+        if (pass_round) pass_round->push_back(rd);
     }
 }
 
@@ -1338,6 +1424,7 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
         }
         b->images.push_back(id);
         b->scenes.push_back(im.scene);
+        b->spp.push_back((uint32_t)im.spp);
         spill_cap = std::max<int64_t>(spill_cap, im.scene->spill_cap);
     }
     b->film_pixels = film;
@@ -1407,12 +1494,91 @@ int spk_batch_destroy(spk_batch* b) {
     return SPK_OK;
 }
 
+int spk_batch_set_guide(spk_batch* b, spk_guide* g) {
+    clear_error();
+    if (!b) return SPK_ERR_INVALID;
+    spk_render* r = b->r;
+    SPK_CHECK_HIP(hipSetDevice(r->device));
+    if (g) {
+        for (spk_scene* sc : b->scenes)
+            if (sc != b->scenes[0]) {
+                set_error("spk_batch_set_guide: a guide serves the images of one scene (image scenes differ)");
+                return SPK_ERR_INVALID;
+            }
+    }
+    int rc;
+    if ((rc = spk_render_set_guide(r, g))) return rc;
+    // re-cut the work: training rounds first (or back to image order without a guide)
+    std::vector<uint32_t> npix;
+    for (const ImageDev& im : b->images) npix.push_back((uint32_t)im.cam.W * (uint32_t)im.cam.H);
+    b->chunks.clear();
+    b->passes.clear();
+    b->pass_round.clear();
+    if (g) {
+        std::vector<int32_t> rounds;
+        batch_chunks_guided(npix, b->spp, r->capacity, (uint32_t)std::max(0, g->opts.train_spp),
+                            (uint32_t)g->opts.train_pass_spp, b->chunks, rounds);
+        batch_passes(b->chunks, r->capacity, b->passes, &rounds, &b->pass_round);
+    } else {
+        for (size_t i = 0; i < b->images.size(); i++) batch_chunks((uint32_t)i, npix[i], b->spp[i], r->capacity, b->chunks);
+        batch_passes(b->chunks, r->capacity, b->passes);
+    }
+    if ((rc = b->chunks_dev.upload(b->chunks.data(), b->chunks.size() * sizeof(ChunkDev))) ||
+        (rc = b->passes_dev.upload(b->passes.data(), b->passes.size() * sizeof(PassDev))))
+        return rc;
+    r->bufs.chunks = b->chunks_dev.as<ChunkDev>();
+    r->bufs.passes = b->passes_dev.as<PassDev>();
+    r->bufs.num_passes = (uint32_t)b->passes.size();
+    if (b->exec) {  // captured with the old tables
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    return SPK_OK;
+}
+
 int spk_batch_run(spk_batch* b, int32_t sync) {
     clear_error();
     if (!b) return SPK_ERR_INVALID;
     spk_render* r = b->r;
     SPK_CHECK_HIP(hipSetDevice(r->device));
     int rc;
+    if (r->guide) {
+        // Guided: passes launched one by one (no graph); a training round's records are turned
+        // into Adam steps after its last pass, and the network stays fixed after the rounds.
+        spk_guide* g = r->guide;
+        if ((rc = launch_clear_film(r->bufs, (uint32_t)b->film_pixels, r->stream))) return rc;
+        if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
+        const SceneDev* sd = &b->scenes[0]->dev;
+        const size_t np = b->passes.size();
+        for (size_t p = 0; p < np; p++) {
+            const int32_t rd = b->pass_round[p];
+            const bool train = rd >= 0 && g->opts.steps_per_pass > 0;
+            if (train && (p == 0 || b->pass_round[p - 1] != rd) &&
+                (rc = zero_u32(g->state.as<uint32_t>(), 2, r->stream)))  // rec_count, rec_fin
+                return rc;
+            GuideNetDev net = guide_net(r, train);
+            net.rec_keep = 1.0f / (float)g->opts.train_pass_spp;
+            if ((rc = launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream, sd, &net))) return rc;
+            if (train && (p + 1 == np || b->pass_round[p + 1] != rd)) {
+                if ((rc = guide_train_round(r, net))) return rc;
+                r->train_passes++;
+            }
+        }
+        r->samples_done++;
+        if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+        return SPK_OK;
+    }
+    if (b->graph && !b->exec) {
+        hipGraph_t gr;
+        SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        int crc = b->graph == 2 ? batch_launch_all(b) : launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream);
+        hipError_t e = hipStreamEndCapture(r->stream, &gr);
+        if (crc) return crc;
+        SPK_CHECK_HIP(e);
+        e = hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        SPK_CHECK_HIP(e);
+    }
     if (b->graph == 2) {
         SPK_CHECK_HIP(hipGraphLaunch(b->exec, r->stream));
     } else if (b->graph == 1) {

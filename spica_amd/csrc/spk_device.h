@@ -160,6 +160,9 @@ struct PassState {
     unsigned long long count_nodes;  // extend traversal node fetches (counting passes)
     unsigned long long count_tris;   // extend traversal triangle tests (counting passes)
     unsigned long long count_rays;   // extend rays traced in counting passes
+    unsigned long long count_nodes_s;  // occlusion traversal node fetches (counting passes)
+    unsigned long long count_tris_s;   // occlusion traversal triangle tests (counting passes)
+    unsigned long long count_rays_s;   // occlusion rays traced in counting passes
 };
 
 }  // namespace spk

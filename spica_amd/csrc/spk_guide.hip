@@ -12,7 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 
@@ -530,11 +530,11 @@ __global__ void k_record_gather(RecordPtrs p, uint32_t cap, const uint32_t* __re
     }
 }
 
-size_t guide_record_sort_bytes(uint32_t cap) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)cap, 0, 64);
-    return bytes;
+int guide_record_sort_bytes(uint32_t cap, size_t* bytes) {
+    *bytes = 0;
+    SPK_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                            (const uint32_t*)nullptr, (uint32_t*)nullptr, cap, 0u, 64u));
+    return SPK_OK;
 }
 
 int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap, const uint32_t* rec_count,
@@ -545,8 +545,8 @@ int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap,
                        (const float4*)rec[4], cap, rec_count, keys, idx);
     SPK_CHECK_HIP(hipGetLastError());
     size_t bytes = tmp_bytes;
-    SPK_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, (const uint64_t*)keys, keys + cap, (const uint32_t*)idx,
-                                                     idx + cap, (int)cap, 0, 64, st));
+    SPK_CHECK_HIP(rocprim::radix_sort_pairs(tmp, bytes, (const uint64_t*)keys, keys + cap, (const uint32_t*)idx, idx + cap,
+                                            cap, 0u, 64u, st));
     RecordPtrs p;
     for (int k = 0; k < 5; k++) {
         p.in[k] = rec[k];

@@ -174,7 +174,9 @@ struct LaunchShape {
 int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls,
                 hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide);
 // One pass of a batch (images, chunks and passes in b); scenes come from b.scenes.
-int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls, hipStream_t st);
+// Guided batches pass the one scene every image renders and the guide network.
+int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls, hipStream_t st,
+                      const SceneDev* guide_scene = nullptr, const GuideNetDev* guide = nullptr);
 int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st);
 int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st);
 int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial,
@@ -214,7 +216,7 @@ int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n
 // Training records of one round in a fixed order: sorted by (sample, pixel, depth) into out[0..4], so
 // the batches, and with them the trained network, do not depend on the order in which the
 // queue appends stored the records.  keys/idx hold 2 * cap entries each (input and sorted).
-size_t guide_record_sort_bytes(uint32_t cap);
+int guide_record_sort_bytes(uint32_t cap, size_t* bytes);
 int sort_guide_records(float4* const rec[5], float4* const out[5], uint32_t cap, const uint32_t* rec_count,
                        uint64_t* keys, uint32_t* idx, void* tmp, size_t tmp_bytes, hipStream_t st);
 int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNetDev* net,

@@ -1692,7 +1692,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
 }
 
 // ---- shadow: NEE light-sample occlusion (visibility_tester.cc:21-24) ------------------------
-template <int BLOCK, int S, bool BATCH>
+template <int BLOCK, int S, bool BATCH, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, int depth) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
@@ -1708,6 +1708,7 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
         tmax = D.w;
         return BATCH ? rb.scenes + __float_as_uint(rb.qs_c[i].w) : nullptr;
     };
+    uint32_t rays = 0;
     auto done = [&](uint32_t i, int32_t k, float, float, float, const SceneDev&) {
         if (k < 0) {
             const uint32_t path = __float_as_uint(rb.qs_o[i].w);
@@ -1715,8 +1716,14 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
             float4 Lv = rb.path_L[path];
             rb.path_L[path] = make_float4(Lv.x + c.x, Lv.y + c.y, Lv.z + c.z, Lv.w);
         }
+        if (COUNT) rays++;
     };
-    traverse_stream<BLOCK, S, true, false, BATCH>(sc, rb.scenes, count, load, done, col, spill, cnt);
+    traverse_stream<BLOCK, S, true, COUNT, BATCH>(sc, rb.scenes, count, load, done, col, spill, cnt);
+    if (COUNT) {
+        atomicAdd(&ps->count_nodes_s, (unsigned long long)cnt.nodes);
+        atomicAdd(&ps->count_tris_s, (unsigned long long)cnt.tris);
+        atomicAdd(&ps->count_rays_s, (unsigned long long)rays);
+    }
 }
 
 // ---- MIS: NEE BSDF-sample ray; counts if it reaches the chosen light (mis.cc:114-133) ----------
@@ -1778,6 +1785,9 @@ __device__ __forceinline__ void guide_inputs(const GuideNetDev& net, float3 p, f
     x[6] = fw.y;
 }
 
+// Batches (BATCH): every image renders one scene (the guide's), queue entries carry the image in
+// the pixel field and the absolute sample index, and each image has its own sampler seed.
+template <bool BATCH>
 __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                       GuideNetDev net, int depth) {
     __shared__ __attribute__((aligned(16))) uint16_t w_lds[kGFwdSize];
@@ -1791,7 +1801,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
     for (int i = threadIdx.x; i < kGBiasSize; i += kGuideBlock) b_lds[i] = net.bias[i];
     __syncthreads();
     const int kind = cfgp->sampler;
-    const uint32_t seed = cfgp->seed;
+    const uint32_t cfg_seed = cfgp->seed;
     const int rr_depth = cfgp->rr_depth;
     const uint32_t sample_base = ps->sample_base;
     const int nxt = (depth & 1) ^ 1;
@@ -1837,16 +1847,18 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         wave_lds_sync();
         bool pushA = false, pushR = false;
         float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0), beta = mk(0, 0, 0);
-        uint32_t nA_flags = 0, path = 0, pixel = 0, rsample = 0;
+        uint32_t nA_flags = 0, path = 0, pfield = 0, rsample = 0;
         if (active) {
             const float* o = ob + lane * kGuideObStride;
             GuideMix mix;
             mix_from_outputs([&](int u) { return o[u]; }, mix);
             path = __float_as_uint(G0.w);
-            pixel = __float_as_uint(G2.w);
+            pfield = __float_as_uint(G2.w);
+            const uint32_t pixel = BATCH ? batch_pixel(pfield) : pfield;
+            const uint32_t seed = BATCH ? rb.images[batch_image(pfield)].seed : cfg_seed;
             beta = xyz(G2);
             const uint32_t flags = __float_as_uint(G1.w);
-            const uint32_t sample = sample_base + (flags >> 7);
+            const uint32_t sample = BATCH ? (flags >> 7) : sample_base + (flags >> 7);
             rsample = sample;
             const uint32_t mbits = __float_as_uint(NG.w);
             const float3 ng = xyz(NG), p = xyz(G0), wo = xyz(G1);
@@ -1886,9 +1898,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                     if (cont) {
                         pushA = true;
                         if (net.record) {
-                            // keyed by (pixel, depth, sample), not the pass-local path slot, so the
-                            // kept records do not depend on how the samples are cut into passes
-                            uint32_t h0 = pixel, h1 = (uint32_t)depth, h2 = sample, h3 = 0x7f4a7c15u;
+                            // keyed by (pixel field, depth, sample), not the pass-local path slot, so
+                            // the kept records do not depend on how the samples are cut into passes
+                            uint32_t h0 = pfield, h1 = (uint32_t)depth, h2 = sample, h3 = 0x7f4a7c15u;
                             pcg4d(h0, h1, h2, h3);
                             pushR = u32_to_unit(h0) < net.rec_keep;
                         }
@@ -1909,7 +1921,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         if (pushA) {
             rb.qa_o[nxt][sa] = make_float4(nA_o.x, nA_o.y, nA_o.z, __uint_as_float(path));
             rb.qa_d[nxt][sa] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float(nA_flags));
-            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pixel));
+            rb.qa_b[nxt][sa] = make_float4(nA_beta.x, nA_beta.y, nA_beta.z, __uint_as_float(pfield));
         }
         if (net.record) {
             const uint32_t sr = sl[1];
@@ -1919,7 +1931,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                 net.rec[1][sr] = make_float4(x[4], x[5], x[6], __uint_as_float(path));
                 net.rec[2][sr] = make_float4(nA_d.x, nA_d.y, nA_d.z, __uint_as_float((uint32_t)depth));
                 net.rec[3][sr] = make_float4(L.x, L.y, L.z, __uint_as_float(rsample));
-                net.rec[4][sr] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(pixel));
+                net.rec[4][sr] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(pfield));
             }
         }
         wave_lds_sync();
@@ -2177,23 +2189,40 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
         if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
         constexpr int SB = kShadeBlock;
         const dim3 gs(ls.grid_stream * 2 * 64 / SB), bs(SB);
-        if (!BATCH && guide) {
-            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, true, false, true>), gs, bs, 0, st, s, b.cfg, b, d);
-            else hipLaunchKernelGGL((k_shade<SB, true, false, false>), gs, bs, 0, st, s, b.cfg, b, d);
+        // profile events per depth d of a pass (nd = maxDepth + 1): closest hit 2d, 2d + 1; guide
+        // 2nd + 2d, + 1; shade 4nd + 2d, + 1; shadow 6nd + 2d, + 1
+        const int nd = cfg.max_depth + 1;
+        auto mark = [&](int e) -> int {
+            if (ev_ext && e < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[e], st));
+            return SPK_OK;
+        };
+        if (mark(4 * nd + 2 * d)) return SPK_ERR_HIP;
+        if (guide) {
+            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, true, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, d);
+            else hipLaunchKernelGGL((k_shade<SB, true, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, d);
         } else {
             if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, false, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, d);
             else hipLaunchKernelGGL((k_shade<SB, false, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, d);
         }
+        if (mark(4 * nd + 2 * d + 1)) return SPK_ERR_HIP;
         if (d < cfg.max_depth) {
-            hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            if (mark(6 * nd + 2 * d)) return SPK_ERR_HIP;
+            if (count)
+                hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, true>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            else
+                hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            if (mark(6 * nd + 2 * d + 1)) return SPK_ERR_HIP;
             hipLaunchKernelGGL((k_mis<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
             // after the NEE kernels, so a training record sees this vertex's direct light
-            if (!BATCH && guide)
-                hipLaunchKernelGGL(k_guide, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
+            if (guide) {
+                if (mark(2 * nd + 2 * d)) return SPK_ERR_HIP;
+                hipLaunchKernelGGL(k_guide<BATCH>, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
+                if (mark(2 * nd + 2 * d + 1)) return SPK_ERR_HIP;
+            }
         }
     }
     hipLaunchKernelGGL(k_film<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
-    if (!BATCH && guide && guide->record) {
+    if (guide && guide->record) {
         hipLaunchKernelGGL(k_guide_finalize, dim3(ls.grid_stream), dim3(256), 0, st, *guide, (const float4*)b.path_L);
         hipLaunchKernelGGL(k_guide_finalize_mark, dim3(1), dim3(64), 0, st, *guide);
     }
@@ -2210,13 +2239,15 @@ int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, co
     }
 }
 
-int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st) {
+int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,
+                      const SceneDev* guide_scene, const GuideNetDev* guide) {
     SceneDev none;
     memset(&none, 0, sizeof(none));
+    const SceneDev& s = guide_scene ? *guide_scene : none;
     switch (ls.block) {
-    case 64: return launch_pass_b<64, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
-    case 256: return launch_pass_b<256, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
-    default: return launch_pass_b<128, true>(none, b, cfg, ls, st, nullptr, 0, false, nullptr);
+    case 64: return launch_pass_b<64, true>(s, b, cfg, ls, st, nullptr, 0, false, guide);
+    case 256: return launch_pass_b<256, true>(s, b, cfg, ls, st, nullptr, 0, false, guide);
+    default: return launch_pass_b<128, true>(s, b, cfg, ls, st, nullptr, 0, false, guide);
     }
 }
 

