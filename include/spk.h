@@ -99,6 +99,10 @@ typedef struct spk_render_opts {
     int32_t use_graph;      /* capture each pass into a hipGraph and replay it */
     int32_t profile;        /* time every traversal launch with HIP events */
     int32_t count_visits;   /* count node fetches / triangle tests of the closest-hit stage */
+    int32_t classic_passes; /* 0 (default): a pass covers up to 2^26 paths and keeps queue_size of
+                             * them in flight, replacing each finished path by the pass's next camera
+                             * path, so every launch sees a full queue; 1: all of a pass's paths
+                             * enter at once and the queues shrink bounce by bounce */
 } spk_render_opts;
 
 /* Creates a render of `scene` through `camera`: film, path queues and stream.

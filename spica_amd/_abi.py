@@ -125,6 +125,7 @@ class SpkRenderOpts(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("profile", ctypes.c_int32),
         ("count_visits", ctypes.c_int32),
+        ("classic_passes", ctypes.c_int32),
     ]
 
 

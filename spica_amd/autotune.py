@@ -1,16 +1,18 @@
 """Autotuner for the wavefront render: paths in flight (queue) and traversal block size.
 
-Deep bounces are latency bound (few long rays per launch), so larger queues amortise
-them over more samples per pass, at the cost of queue memory (~224 B per path).
-The tuner times two full passes per candidate (a probe shorter than a pass would
-never fill a large queue and understate it), keeps the fastest by traced rays per
-second and caches the choice per (scene size, film, depth, device).
+Passes regenerate paths: a pass covers up to 2^26 paths (whole samples of the film) and keeps
+`queue` of them in flight, starting the pass's next camera path in every slot a finished path
+frees, so every launch works on a full queue of mixed-depth paths.  More paths in flight hide
+more latency per launch at ~272 B of queue memory each; the candidates span 2^20..2^23.
+The tuner times two full passes per candidate, keeps the fastest by traced rays per second and
+caches the choice per (scene size, film, depth, device).
 """
 import json
 import os
 import time
 
-DEFAULT_QUEUES = (1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
+DEFAULT_QUEUES = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+PASS_PATHS = 1 << 26  # paths per regenerating pass
 DEFAULT_BLOCKS = (64, 128, 256)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "spica_amd", "autotune.json")
 
@@ -19,7 +21,8 @@ def _key(gscene, camera, desc):
     import torch
 
     st = gscene.stats()
-    return f"{torch.cuda.get_device_name(0)}|tris={st['num_tris']}|{camera.width}x{camera.height}|depth={desc.max_depth}"
+    return (f"{torch.cuda.get_device_name(0)}|tris={st['num_tris']}|{camera.width}x{camera.height}|depth={desc.max_depth}"
+            f"|regen")
 
 
 def _load():
@@ -54,11 +57,9 @@ def tune(gscene, camera, desc, queues=DEFAULT_QUEUES, blocks=DEFAULT_BLOCKS, spp
     npix = camera.width * camera.height
     table = []
     best = None
+    per_pass = max(1, PASS_PATHS // npix)
     for q in queues:
-        per_pass = max(1, q // npix)
-        n = max(spp, 2 * per_pass)   # two full passes
-        if n > max_spp and q > queues[0]:
-            continue
+        n = min(max(spp, 2 * per_pass), max_spp)  # two full passes where they fit in max_spp
         for b in blocks:
             gr = render.GpuRender(gscene, camera, desc, queue_size=q, block_size=b)
             gr.run(per_pass)

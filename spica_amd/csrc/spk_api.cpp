@@ -50,6 +50,15 @@ void DevBuf::release() {
 }
 
 int zero_u32(uint32_t* p, int n, hipStream_t st);
+
+// Paths per regenerating pass (one path radiance slot each, 16 bytes): the in-flight queues are
+// the render's queue size, refilled from the pass's unstarted paths.
+#ifndef SPK_REGEN_PASS_LOG2
+#define SPK_REGEN_PASS_LOG2 26
+#endif
+constexpr int64_t kRegenPassPaths = int64_t(1) << SPK_REGEN_PASS_LOG2;
+// Iterations of a regenerating pass launched between polls of its done flag.
+constexpr int kRegenGroup = 4;
 int launch_sampler(int kind, uint32_t seed, int64_t n, const uint32_t* px, const uint32_t* sm, const uint32_t* dm,
                    uint32_t* out, hipStream_t st);
 
@@ -258,7 +267,12 @@ struct spk_render {
     LaunchShape shape;
     DevBuf qa_o[2], qa_d[2], qa_b[2], hit, qs_o, qs_d, qs_c, qm_o, qm_d, qm_c, path_L;
     DevBuf film_sum, film_c, film_sq, film_sqc, spill, cfg_dev, ps_dev, scratch;
-    uint32_t capacity = 0;
+    uint32_t capacity = 0;          // paths per pass (path radiance slots)
+    uint32_t inflight = 0;          // queue capacity: = capacity for classic passes
+    uint32_t* rg_flag = nullptr;    // regen: pinned host words the done flag is copied to
+    hipEvent_t rg_ev[2] = {nullptr, nullptr};
+    hipGraphExec_t rg_exec = nullptr;  // use_graph with regen: one captured group of iterations
+    int64_t rg_iterations = 0;      // regen iterations launched
     int64_t samples_done = 0;
     double extend_ms = 0.0;
     int64_t extend_launches = 0;
@@ -285,6 +299,10 @@ struct spk_render {
         }
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (pass_exec) (void)hipGraphExecDestroy(pass_exec);
+        if (rg_exec) (void)hipGraphExecDestroy(rg_exec);
+        for (hipEvent_t e : rg_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (rg_flag) (void)hipHostFree(rg_flag);
     }
 };
 
@@ -795,7 +813,10 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     c.W = cam->width;
     c.H = cam->height;
     c.npix = (uint32_t)cam->width * (uint32_t)cam->height;
-    uint32_t qcap = (uint32_t)o.queue_size;
+    // Regenerating passes (the default) cover up to kRegenPassPaths paths each with queue_size
+    // paths in flight; classic passes hold all of their paths in the queues at once.
+    const bool regen = o.classic_passes == 0;
+    uint32_t qcap = regen ? (uint32_t)std::max<int64_t>(o.queue_size, kRegenPassPaths) : (uint32_t)o.queue_size;
     if (c.npix <= qcap) {
         // samples per pass; bounded so the per-path sample slot fits the queue flags
         // (bits 7..31) and the film kernel's per-pixel loop stays short
@@ -808,6 +829,9 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
         r->capacity = qcap;
     }
     c.queue = c.chunks > 1 ? qcap : r->capacity;
+    r->inflight = regen ? (uint32_t)std::min<int64_t>(o.queue_size, r->capacity) : r->capacity;
+    c.regen = regen ? 1 : 0;
+    c.inflight = r->inflight;
     c.max_depth = desc->max_depth;
     c.rr_depth = desc->rr_depth;
     c.sampler = desc->sampler;
@@ -815,16 +839,21 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     c.integrator = desc->integrator;
     c.wide = s->dev.wide;
 
-    // Buffers.
+    // Buffers: queues hold the paths in flight, path radiance every path of a pass.
     int rc;
-    size_t Q = r->capacity;
+    size_t Q = r->inflight;
     for (int k = 0; k < 2; k++) {
         if ((rc = r->qa_o[k].alloc(Q * 16)) || (rc = r->qa_d[k].alloc(Q * 16)) || (rc = r->qa_b[k].alloc(Q * 16))) return rc;
     }
     if ((rc = r->hit.alloc(Q * 16)) || (rc = r->qs_o.alloc(Q * 16)) || (rc = r->qs_d.alloc(Q * 16)) ||
         (rc = r->qs_c.alloc(Q * 16)) || (rc = r->qm_o.alloc(Q * 16)) || (rc = r->qm_d.alloc(Q * 16)) ||
-        (rc = r->qm_c.alloc(Q * 16)) || (rc = r->path_L.alloc(Q * 16)))
+        (rc = r->qm_c.alloc(Q * 16)) || (rc = r->path_L.alloc((size_t)r->capacity * 16)))
         return rc;
+    if (regen) {
+        SPK_CHECK_HIP(hipHostMalloc((void**)&r->rg_flag, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        r->rg_flag[0] = r->rg_flag[1] = 0;
+        for (hipEvent_t& e : r->rg_ev) SPK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     size_t P = c.npix;
     if ((rc = r->film_sum.alloc(P * 16)) || (rc = r->film_c.alloc(P * 16)) || (rc = r->film_sq.alloc(P * 16)) ||
         (rc = r->film_sqc.alloc(P * 16)))
@@ -924,6 +953,102 @@ int guide_train_round(spk_render* r, GuideNetDev net) {
     return SPK_OK;
 }
 
+// Profile events of regenerating passes: 8 per iteration (closest hit, guide, shade, shadow).
+int ensure_events(spk_render* r, size_t need) {
+    while (r->events.size() < need) {
+        hipEvent_t e;
+        SPK_CHECK_HIP(hipEventCreate(&e));
+        r->events.push_back(e);
+    }
+    return SPK_OK;
+}
+
+// One regenerating pass: set-up, iterations in groups of kRegenGroup until one finds no work,
+// then the film.  The done flag is copied to pinned host memory after every group and read one
+// group later, so the host never stalls the stream (at most one group of empty iterations runs
+// at the end).  With use_graph one captured group (an even number of iterations, so every group
+// has the same queue slots) is replayed.  Profile events are appended from index *ev_next.
+int run_pass_regen(spk_render* r, const SceneDev& sd, bool batch, bool use_graph, const GuideNetDev* net,
+                   size_t* ev_next) {
+    int rc;
+    if ((rc = launch_regen_begin(r->bufs, batch, r->stream))) return rc;
+    const bool prof = r->opts.profile != 0 && ev_next;
+    const bool count = r->opts.count_visits != 0;
+    static_assert(kRegenGroup % 2 == 0, "a group keeps the slot pattern");
+    // Every queued path advances one bounce per iteration and ends after maxDepth + 1, and an
+    // iteration starts new paths whenever the queue has room, so a pass needs at most this many
+    // iterations; beyond it something is wrong and the pass is abandoned with an error.
+    const int64_t cap_paths = batch ? (int64_t)r->capacity : (int64_t)std::max<uint32_t>(r->capacity, 1);
+    const int64_t max_it = (cap_paths / std::max<uint32_t>(r->inflight, 1) + 2) * (r->cfg.max_depth + 2) +
+                           4 * kRegenGroup;
+    int64_t it = 0;
+    for (int g = 0;; g++) {
+        if (it > max_it) {
+            SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+            set_error("regenerating pass did not finish after %lld iterations", (long long)it);
+            return SPK_ERR_STATE;
+        }
+        if (use_graph && !prof && !net) {
+            if (!r->rg_exec) {
+                hipGraph_t gr;
+                SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+                int crc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, 0, kRegenGroup, nullptr,
+                                             count, nullptr);
+                hipError_t e = hipStreamEndCapture(r->stream, &gr);
+                if (crc) return crc;
+                SPK_CHECK_HIP(e);
+                e = hipGraphInstantiate(&r->rg_exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                SPK_CHECK_HIP(e);
+            }
+            SPK_CHECK_HIP(hipGraphLaunch(r->rg_exec, r->stream));
+        } else {
+            hipEvent_t* ev = nullptr;
+            if (prof) {
+                if ((rc = ensure_events(r, *ev_next + 8 * kRegenGroup))) return rc;
+                ev = &r->events[*ev_next];
+                *ev_next += 8 * kRegenGroup;
+            }
+            if ((rc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, (int)it, kRegenGroup, ev, count,
+                                         net)))
+                return rc;
+        }
+        it += kRegenGroup;
+        SPK_CHECK_HIP(hipMemcpyAsync(&r->rg_flag[g & 1], &r->bufs.ps->rg_done, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     r->stream));
+        SPK_CHECK_HIP(hipEventRecord(r->rg_ev[g & 1], r->stream));
+        if (g >= 1) {
+            SPK_CHECK_HIP(hipEventSynchronize(r->rg_ev[(g - 1) & 1]));
+            if (r->rg_flag[(g - 1) & 1]) break;
+        }
+    }
+    r->rg_iterations += it;
+    return launch_regen_end(r->bufs, r->cfg, r->shape, r->stream, batch, (int)((it - 1) & 1), net);
+}
+
+// Adds the profile events of iterations [0, n_it) stored from index 0 (8 per iteration).
+int regen_profile(spk_render* r, int64_t n_it, bool guided) {
+    for (int64_t k = 0; k < n_it; k++) {
+        float ms = 0.0f;
+        hipEvent_t* e = &r->events[(size_t)k * 8];
+        SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        r->extend_ms += ms;
+        r->extend_launches++;
+        if (guided) {
+            SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[2], e[3]));
+            r->guide_ms += ms;
+            r->guide_launches++;
+        }
+        SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[4], e[5]));
+        r->shade_ms += ms;
+        r->shade_launches++;
+        SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[6], e[7]));
+        r->shadow_ms += ms;
+        r->shadow_launches++;
+    }
+    return SPK_OK;
+}
+
 // Samples [first, first + n_spp) as passes of the compiled pipeline.
 int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* net) {
     ConfigDev& c = r->cfg;
@@ -943,12 +1068,21 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
         SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
         ext_before = (int64_t)ps.total_extend;
         shadow_before = (int64_t)ps.total_shadow;
-        size_t need = (size_t)passes * nev;
-        while (r->events.size() < need) {
-            hipEvent_t e;
-            SPK_CHECK_HIP(hipEventCreate(&e));
-            r->events.push_back(e);
+        if (!c.regen && (rc = ensure_events(r, (size_t)passes * nev))) return rc;
+    }
+    if (c.regen) {
+        size_t ev_next = 0;
+        for (int64_t p = 0; p < passes; p++)
+            if ((rc = run_pass_regen(r, r->scene->dev, false, r->opts.use_graph != 0, net, &ev_next))) return rc;
+        if (r->opts.profile) {
+            SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+            PassState ps;
+            SPK_CHECK_HIP(hipMemcpy(&ps, r->ps_dev.p, sizeof(PassState), hipMemcpyDeviceToHost));
+            r->extend_rays_timed += (int64_t)ps.total_extend - ext_before;
+            r->shadow_rays_timed += (int64_t)ps.total_shadow - shadow_before;
+            if ((rc = regen_profile(r, (int64_t)(ev_next / 8), net != nullptr))) return rc;
         }
+        return SPK_OK;
     }
     if (r->opts.use_graph && !r->opts.profile && !net) {
         // The pass reads its sample range from device memory, so one capture serves every pass.
@@ -1075,7 +1209,7 @@ int spk_render_set_guide(spk_render* r, spk_guide* g) {
     r->guide_trained = 0;
     if (!g || r->qg[0].p) return SPK_OK;
     int rc;
-    const size_t Q = r->capacity;
+    const size_t Q = r->inflight;
     for (int k = 0; k < 4; k++)
         if ((rc = r->qg[k].alloc(Q * 16))) return rc;
     // training records of one round: about one kept vertex per pixel, room for four
@@ -1472,7 +1606,7 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
     r->bufs.num_passes = (uint32_t)b->passes.size();
     r->bufs.spill_cap = (int)spill_cap;
     SPK_CHECK_HIP(hipMemcpy(r->cfg_dev.p, &r->cfg, sizeof(ConfigDev), hipMemcpyHostToDevice));
-    if (b->graph) {
+    if (b->graph && !r->cfg.regen) {
         // Captured work: kernel launches only (no memset nodes).
         hipGraph_t g;
         SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
@@ -1558,12 +1692,27 @@ int spk_batch_run(spk_batch* b, int32_t sync) {
                 return rc;
             GuideNetDev net = guide_net(r, train);
             net.rec_keep = 1.0f / (float)g->opts.train_pass_spp;
-            if ((rc = launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream, sd, &net))) return rc;
+            if (r->cfg.regen) rc = run_pass_regen(r, *sd, true, false, &net, nullptr);
+            else rc = launch_batch_pass(r->bufs, r->cfg, r->shape, r->stream, sd, &net);
+            if (rc) return rc;
             if (train && (p + 1 == np || b->pass_round[p + 1] != rd)) {
                 if ((rc = guide_train_round(r, net))) return rc;
                 r->train_passes++;
             }
         }
+        r->samples_done++;
+        if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
+        return SPK_OK;
+    }
+    if (r->cfg.regen) {
+        // regenerating passes, each driven to completion; graph modes replay captured groups of
+        // iterations (a whole-batch graph would need the iteration counts in advance)
+        SceneDev none;
+        memset(&none, 0, sizeof(none));
+        if ((rc = launch_clear_film(r->bufs, (uint32_t)b->film_pixels, r->stream))) return rc;
+        if ((rc = launch_set_pass(r->bufs.ps, 0, r->stream))) return rc;
+        for (size_t p = 0; p < b->passes.size(); p++)
+            if ((rc = run_pass_regen(r, none, true, b->graph != 0, nullptr, nullptr))) return rc;
         r->samples_done++;
         if (sync) SPK_CHECK_HIP(hipStreamSynchronize(r->stream));
         return SPK_OK;
@@ -1629,7 +1778,7 @@ int spk_batch_info_get(spk_batch* b, spk_batch_info* out) {
     out->chunks = (int64_t)b->chunks.size();
     out->paths = b->paths;
     out->film_pixels = b->film_pixels;
-    out->queue = b->r->capacity;
+    out->queue = b->r->inflight;
     return SPK_OK;
 }
 

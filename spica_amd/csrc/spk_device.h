@@ -128,6 +128,8 @@ struct ConfigDev {
     uint32_t seed;
     int integrator;         // SPK_INTEGRATOR_PATH / SPK_INTEGRATOR_DIRECT
     int wide;               // some scene needs the wide-material shading kernel
+    int regen;              // spent paths are replaced by the pass's next camera paths
+    uint32_t inflight;      // regen: queue capacity (paths in flight); the pass holds up to `queue`
 };
 
 // Device-side per-pass state: queue lengths and work counters per bounce.
@@ -163,6 +165,10 @@ struct PassState {
     unsigned long long count_nodes_s;  // occlusion traversal node fetches (counting passes)
     unsigned long long count_tris_s;   // occlusion traversal triangle tests (counting passes)
     unsigned long long count_rays_s;   // occlusion rays traced in counting passes
+    // regen: per iteration, the continuations carried over (n_cont), the new camera paths
+    // appended after them (m_new, path ids from first_new), the next unstarted path of the
+    // pass, the extend count, and done = no work left in this iteration
+    uint32_t rg_next, rg_ncont, rg_mnew, rg_first, rg_nin, rg_done, rg_pad[2];
 };
 
 }  // namespace spk

@@ -177,6 +177,14 @@ int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg_hos
 // Guided batches pass the one scene every image renders and the guide network.
 int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg_host, const LaunchShape& ls, hipStream_t st,
                       const SceneDev* guide_scene = nullptr, const GuideNetDev* guide = nullptr);
+// Regenerating passes, driven from the host: pass set-up, n_it iterations starting at
+// iteration it0 (queue slot it & 1; ev: 8 events per iteration or null), and the film.
+int launch_regen_begin(const RenderBufs& b, bool batch, hipStream_t st);
+int launch_regen_iters(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, bool count,
+                       const GuideNetDev* guide);
+int launch_regen_end(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st, bool batch,
+                     int last_slot, const GuideNetDev* guide);
 int launch_set_pass(PassState* ps, uint32_t index, hipStream_t st);
 int launch_clear_film(const RenderBufs& b, uint32_t npix, hipStream_t st);
 int launch_relmse(const RenderBufs& b, const ConfigDev& cfg, const double* ref, double eps, double* out_partial,

@@ -555,7 +555,46 @@ __global__ void k_pass_init(const ConfigDev* __restrict__ cfg, PassState* ps, co
         ps->countG[i] = 0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) ps->countA[0] = n_sh;
+    if (threadIdx.x == 0) {
+        // classic passes start with all their camera paths in the queue; regenerating passes
+        // with an empty one that k_plan fills
+        ps->countA[0] = cfg->regen ? 0u : n_sh;
+        ps->rg_next = 0;
+        ps->rg_nin = 0;
+        ps->rg_ncont = 0;
+        ps->rg_mnew = 0;
+        ps->rg_first = 0;
+        ps->rg_done = 0;
+    }
+}
+
+// One iteration of a regenerating pass, before its kernels (one thread).  The queue in `slot`
+// holds the continuations the previous iteration produced; the pass's next unstarted camera
+// paths fill it up to the in-flight capacity.  The previous iteration's queue counts go to the
+// totals (the last iteration's are added by k_film).
+__global__ void k_plan(const ConfigDev* __restrict__ cfg, PassState* ps, int slot) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int o = slot ^ 1;
+    ps->total_extend += ps->rg_nin;
+    ps->total_shadow += ps->countS[o];
+    ps->total_mis += ps->countM[o];
+    ps->total_guided += ps->countG[o];
+    ps->countS[o] = 0;
+    ps->countM[o] = 0;
+    ps->countG[o] = 0;
+    const uint32_t ncont = ps->countA[slot];
+    const uint32_t next = ps->rg_next, total = ps->done ? 0u : ps->n;
+    const uint32_t room = cfg->inflight > ncont ? cfg->inflight - ncont : 0u;
+    const uint32_t m = min(room, total - min(next, total));
+    ps->rg_ncont = ncont;
+    ps->rg_mnew = m;
+    ps->rg_first = next;
+    ps->rg_next = next + m;
+    ps->rg_nin = ncont + m;
+    ps->countA[slot] = ncont + m;
+    ps->countA[o] = 0;
+    ps->total_paths += m;
+    ps->rg_done = (ncont + m == 0) ? 1u : 0u;
 }
 
 // Last chunk c in [b, e) whose offset field (path_off or pix_off) is <= i.
@@ -588,13 +627,19 @@ __device__ __forceinline__ void swizzle(uint32_t q, int W, int H, int& x, int& y
 
 // ---- camera (cameras/perspective.cc:53-74, integrator.cc:84-86) ---------------------------
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cfg, RenderBufs rb) {
+__global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cfg, RenderBufs rb, int slot) {
     const PassState* ps = rb.ps;
-    const uint32_t n = ps->n;
+    // classic: the pass's paths 0 .. n-1 at queue positions 0 .. n-1; regen: paths first ..
+    // first + n - 1 appended after the n_cont continuations of queue `slot`
+    const bool rg = cfg->regen != 0;
+    const uint32_t n = rg ? ps->rg_mnew : ps->n;
+    const uint32_t first = rg ? ps->rg_first : 0u, dst0 = rg ? ps->rg_ncont : 0u;
+    const int q = rg ? (slot & 1) : 0;
     const uint32_t n_pix = ps->n_pix, pixel_base = ps->pixel_base, sample_base = ps->sample_base;
     const uint32_t cb = ps->chunk_begin, ce = ps->chunk_end;
     const int kind = cfg->sampler;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t i = first + t;  // path id within the pass
         int x, y;
         uint32_t pixel, sample, flags, field;
         CameraDev cam;
@@ -653,25 +698,25 @@ __global__ __launch_bounds__(256) void k_camera(const ConfigDev* __restrict__ cf
                        m[8] * org.x + m[9] * org.y + m[10] * org.z + m[11]);
         float3 dw = normalize(mk(m[0] * dir.x + m[1] * dir.y + m[2] * dir.z, m[4] * dir.x + m[5] * dir.y + m[6] * dir.z,
                                  m[8] * dir.x + m[9] * dir.y + m[10] * dir.z));
-        rb.qa_o[0][i] = make_float4(ow.x, ow.y, ow.z, __uint_as_float(i));
-        rb.qa_d[0][i] = make_float4(dw.x, dw.y, dw.z, __uint_as_float(flags));
-        rb.qa_b[0][i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(field));
+        rb.qa_o[q][dst0 + t] = make_float4(ow.x, ow.y, ow.z, __uint_as_float(i));
+        rb.qa_d[q][dst0 + t] = make_float4(dw.x, dw.y, dw.z, __uint_as_float(flags));
+        rb.qa_b[q][dst0 + t] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(field));
         rb.path_L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
 // ---- extend: closest hit over the extend queue -------------------------------------------------
 template <int BLOCK, int S, bool COUNT, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversal_waves<BATCH, COUNT>()))) void k_extend(SceneDev sc, RenderBufs rb, int depth) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(traversal_waves<BATCH, COUNT>()))) void k_extend(SceneDev sc, RenderBufs rb, int slot) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
-    const uint32_t count = ps->countA[depth];
+    const uint32_t count = ps->countA[slot];
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
-    const float4* __restrict__ qo = rb.qa_o[depth & 1];
-    const float4* __restrict__ qd = rb.qa_d[depth & 1];
-    const float4* __restrict__ qb = rb.qa_b[depth & 1];
+    const float4* __restrict__ qo = rb.qa_o[slot & 1];
+    const float4* __restrict__ qd = rb.qa_d[slot & 1];
+    const float4* __restrict__ qb = rb.qa_b[slot & 1];
     Counters cnt = {0, 0};
     uint32_t rays = 0;
     auto load = [&](uint32_t i, float3& o, float3& d, float& tmax) -> const SceneDev* {
@@ -1453,9 +1498,9 @@ constexpr int kShadeBlock = 256;
 
 template <int BLOCK, bool GUIDED, bool BATCH, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
-                                                 int depth) {
+                                                 int slot, int oslot) {
     PassState* ps = rb.ps;
-    const uint32_t count = ps->countA[depth];
+    const uint32_t count = ps->countA[slot];
     if (count == 0) return;
     const int kind = cfgp->sampler;
     const uint32_t cfg_seed = cfgp->seed;
@@ -1464,8 +1509,7 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
     // recursion only through specular reflection (see the continuation below)
     const bool direct = cfgp->integrator == SPK_INTEGRATOR_DIRECT;
     const uint32_t sample_base = ps->sample_base;
-    const int cur = depth & 1, nxt = cur ^ 1;
-    const uint32_t dim_base = direct ? 4u + 10u * (uint32_t)depth : 4u + 8u * (uint32_t)depth;
+    const int cur = slot & 1, nxt = oslot & 1;
     __shared__ uint32_t app_lds[2 * 4 * (BLOCK / 64 + 1)];
     uint32_t parity = 0;
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
@@ -1486,6 +1530,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
             path = __float_as_uint(O.w);
             const uint32_t flags = __float_as_uint(D.w);
             const bool specular = (flags >> 6) & 1u;
+            // the vertex's bounce count travels in the entry (bits 0..5): one queue may hold
+            // paths of every depth when spent paths are replaced by new camera paths
+            const int depth = (int)(flags & 0x3fu);
+            const uint32_t dim_base = direct ? 4u + 10u * (uint32_t)depth : 4u + 8u * (uint32_t)depth;
             // batch entries carry the image in the pixel field and the absolute sample
             pfield = __float_as_uint(B.w);
             img = BATCH ? batch_image(pfield) : 0u;
@@ -1652,14 +1700,14 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
         uint32_t* ctr[NQ];
         uint32_t sl[NQ];
         pr[0] = pushA;
-        ctr[0] = &ps->countA[depth + 1];
+        ctr[0] = &ps->countA[oslot];
         pr[1] = pushS;
-        ctr[1] = &ps->countS[depth];
+        ctr[1] = &ps->countS[slot];
         pr[2] = pushM;
-        ctr[2] = &ps->countM[depth];
+        ctr[2] = &ps->countM[slot];
         if constexpr (GUIDED) {
             pr[NQ - 1] = pushG;
-            ctr[NQ - 1] = &ps->countG[depth];
+            ctr[NQ - 1] = &ps->countG[slot];
         }
         block_append<BLOCK / 64, NQ>(pr, ctr, sl, app_lds, parity);
         parity ^= 1u;
@@ -1693,10 +1741,10 @@ __global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDe
 
 // ---- shadow: NEE light-sample occlusion (visibility_tester.cc:21-24) ------------------------
 template <int BLOCK, int S, bool BATCH, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, int depth) {
+__global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, int slot) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
-    const uint32_t count = ps->countS[depth];
+    const uint32_t count = ps->countS[slot];
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
@@ -1728,10 +1776,10 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, RenderBufs rb, in
 
 // ---- MIS: NEE BSDF-sample ray; counts if it reaches the chosen light (mis.cc:114-133) ----------
 template <int BLOCK, int S, bool BATCH>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BATCH ? 7 : 8))) void k_mis(SceneDev sc, RenderBufs rb, int depth) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BATCH ? 7 : 8))) void k_mis(SceneDev sc, RenderBufs rb, int slot) {
     __shared__ uint32_t lds[S * BLOCK];
     PassState* ps = rb.ps;
-    const uint32_t count = ps->countM[depth];
+    const uint32_t count = ps->countM[slot];
     if (count == 0) return;
     uint32_t* col = lds + threadIdx.x;
     uint32_t* spill = rb.spill + (size_t)(blockIdx.x * BLOCK + threadIdx.x) * rb.spill_cap;
@@ -1789,12 +1837,12 @@ __device__ __forceinline__ void guide_inputs(const GuideNetDev& net, float3 p, f
 // the pixel field and the absolute sample index, and each image has its own sampler seed.
 template <bool BATCH>
 __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
-                                                      GuideNetDev net, int depth) {
+                                                      GuideNetDev net, int slot, int oslot) {
     __shared__ __attribute__((aligned(16))) uint16_t w_lds[kGFwdSize];
     __shared__ float b_lds[kGBiasSize];
     __shared__ float obuf[kGuideBlock / 64][64 * kGuideObStride];
     PassState* ps = rb.ps;
-    const uint32_t count = ps->countG[depth];
+    const uint32_t count = ps->countG[slot];
     if (count == 0) return;
     for (int i = threadIdx.x; i < kGFwdSize / 8; i += kGuideBlock)
         reinterpret_cast<u32x4_t*>(w_lds)[i] = reinterpret_cast<const u32x4_t*>(net.wf)[i];
@@ -1804,8 +1852,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
     const uint32_t cfg_seed = cfgp->seed;
     const int rr_depth = cfgp->rr_depth;
     const uint32_t sample_base = ps->sample_base;
-    const int nxt = (depth & 1) ^ 1;
-    const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
+    const int nxt = oslot & 1;
     const int lane = __lane_id(), wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     float* ob = obuf[wv];
     __shared__ uint32_t app_lds[2 * 2 * (kGuideBlock / 64 + 1)];
@@ -1848,6 +1895,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         bool pushA = false, pushR = false;
         float3 nA_o = mk(0, 0, 0), nA_d = mk(0, 0, 1), nA_beta = mk(0, 0, 0), beta = mk(0, 0, 0);
         uint32_t nA_flags = 0, path = 0, pfield = 0, rsample = 0;
+        int depth = 0;
         if (active) {
             const float* o = ob + lane * kGuideObStride;
             GuideMix mix;
@@ -1858,6 +1906,8 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             const uint32_t seed = BATCH ? rb.images[batch_image(pfield)].seed : cfg_seed;
             beta = xyz(G2);
             const uint32_t flags = __float_as_uint(G1.w);
+            depth = (int)(flags & 0x3fu);
+            const uint32_t dim_base = 4u + 8u * (uint32_t)depth;
             const uint32_t sample = BATCH ? (flags >> 7) : sample_base + (flags >> 7);
             rsample = sample;
             const uint32_t mbits = __float_as_uint(NG.w);
@@ -1913,7 +1963,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             }
         }
         const bool pr[2] = {pushA, pushR};
-        uint32_t* const ctr[2] = {&ps->countA[depth + 1], net.rec_count};
+        uint32_t* const ctr[2] = {&ps->countA[oslot], net.rec_count};
         uint32_t sl[2];
         block_append<kGuideBlock / 64, 2>(pr, ctr, sl, app_lds, parity);
         parity ^= 1u;
@@ -1962,7 +2012,8 @@ __device__ __forceinline__ void kahan(float& s, float& c, float x) {
     s = t;
 }
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg, RenderBufs rb, int max_depth) {
+__global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg, RenderBufs rb, int max_depth,
+                                             int last_slot) {
     PassState* ps = rb.ps;
     const uint32_t n_items = BATCH ? ps->n_pixels : ps->n_pix;
     const uint32_t cb = ps->chunk_begin, ce = ps->chunk_end;
@@ -2009,7 +2060,14 @@ __global__ __launch_bounds__(256) void k_film(const ConfigDev* __restrict__ cfg,
         rb.film_sq[f] = q;
         rb.film_sqc[f] = qc;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !ps->done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !ps->done && cfg->regen) {
+        // the last iteration's queue counts (k_plan added the earlier ones and the paths)
+        ps->total_extend += ps->rg_nin;
+        ps->total_shadow += ps->countS[last_slot];
+        ps->total_mis += ps->countM[last_slot];
+        ps->total_guided += ps->countG[last_slot];
+        ps->total_samples += ps->k_samples;
+    } else if (blockIdx.x == 0 && threadIdx.x == 0 && !ps->done) {
         unsigned long long e = 0, sh = 0, mi = 0, gu = 0;
         for (int d = 0; d <= max_depth; d++) {
             e += ps->countA[d];
@@ -2175,58 +2233,95 @@ int launch_relmse_pixels(const RenderBufs& b, int64_t n, const int32_t* pixels, 
     return SPK_OK;
 }
 
+// The stage kernels of one bounce (classic: depth d, queue slot d, output slot d + 1) or of one
+// iteration of a regenerating pass (queue slot it & 1, output slot the other one, preceded by
+// k_plan and the camera kernel that refills the queue).  Profile events per step (e0 = the
+// step's first event; nd = maxDepth + 1 for a classic pass, 1 for an iteration): closest hit
+// e0 + 2d, + 1; guide e0 + 2nd + 2d, + 1; shade e0 + 4nd + 2d, + 1; shadow e0 + 6nd + 2d, + 1.
 template <int B, bool BATCH>
-static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
-                         hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
-    hipLaunchKernelGGL(k_pass_init<BATCH>, dim3(1), dim3(256), 0, st, b.cfg, b.ps, b.passes, b.num_passes);
-    hipLaunchKernelGGL(k_camera<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b);
-    for (int d = 0; d <= cfg.max_depth; d++) {
-        if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d], st));
+static int launch_stages(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                         hipStream_t st, int slot, int oslot, int d, int nd, bool last, hipEvent_t* ev, int n_ev,
+                         bool count, const GuideNetDev* guide) {
+    auto mark = [&](int e) -> int {
+        if (ev && e < n_ev) SPK_CHECK_HIP(hipEventRecord(ev[e], st));
+        return SPK_OK;
+    };
+    if (mark(2 * d)) return SPK_ERR_HIP;
+    if (count)
+        hipLaunchKernelGGL((k_extend<B, kStackSlots, true, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, slot);
+    else
+        hipLaunchKernelGGL((k_extend<B, kStackSlots, false, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, slot);
+    if (mark(2 * d + 1)) return SPK_ERR_HIP;
+    constexpr int SB = kShadeBlock;
+    const dim3 gs(ls.grid_stream * 2 * 64 / SB), bs(SB);
+    if (mark(4 * nd + 2 * d)) return SPK_ERR_HIP;
+    if (guide) {
+        if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, true, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, slot, oslot);
+        else hipLaunchKernelGGL((k_shade<SB, true, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, slot, oslot);
+    } else {
+        if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, false, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, slot, oslot);
+        else hipLaunchKernelGGL((k_shade<SB, false, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, slot, oslot);
+    }
+    if (mark(4 * nd + 2 * d + 1)) return SPK_ERR_HIP;
+    if (!last) {
+        if (mark(6 * nd + 2 * d)) return SPK_ERR_HIP;
         if (count)
-            hipLaunchKernelGGL((k_extend<B, kStackSlots, true, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
+            hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, true>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, slot);
         else
-            hipLaunchKernelGGL((k_extend<B, kStackSlots, false, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
-        if (ev_ext && 2 * d + 1 < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[2 * d + 1], st));
-        constexpr int SB = kShadeBlock;
-        const dim3 gs(ls.grid_stream * 2 * 64 / SB), bs(SB);
-        // profile events per depth d of a pass (nd = maxDepth + 1): closest hit 2d, 2d + 1; guide
-        // 2nd + 2d, + 1; shade 4nd + 2d, + 1; shadow 6nd + 2d, + 1
-        const int nd = cfg.max_depth + 1;
-        auto mark = [&](int e) -> int {
-            if (ev_ext && e < n_ev_ext) SPK_CHECK_HIP(hipEventRecord(ev_ext[e], st));
-            return SPK_OK;
-        };
-        if (mark(4 * nd + 2 * d)) return SPK_ERR_HIP;
+            hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, slot);
+        if (mark(6 * nd + 2 * d + 1)) return SPK_ERR_HIP;
+        hipLaunchKernelGGL((k_mis<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, slot);
+        // after the NEE kernels, so a training record sees this vertex's direct light
         if (guide) {
-            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, true, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, d);
-            else hipLaunchKernelGGL((k_shade<SB, true, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, d);
-        } else {
-            if (cfg.wide) hipLaunchKernelGGL((k_shade<SB, false, BATCH, true>), gs, bs, 0, st, s, b.cfg, b, d);
-            else hipLaunchKernelGGL((k_shade<SB, false, BATCH, false>), gs, bs, 0, st, s, b.cfg, b, d);
-        }
-        if (mark(4 * nd + 2 * d + 1)) return SPK_ERR_HIP;
-        if (d < cfg.max_depth) {
-            if (mark(6 * nd + 2 * d)) return SPK_ERR_HIP;
-            if (count)
-                hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, true>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
-            else
-                hipLaunchKernelGGL((k_shadow<B, kStackSlots, BATCH, false>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
-            if (mark(6 * nd + 2 * d + 1)) return SPK_ERR_HIP;
-            hipLaunchKernelGGL((k_mis<B, kStackSlots, BATCH>), dim3(ls.grid_trace), dim3(B), 0, st, s, b, d);
-            // after the NEE kernels, so a training record sees this vertex's direct light
-            if (guide) {
-                if (mark(2 * nd + 2 * d)) return SPK_ERR_HIP;
-                hipLaunchKernelGGL(k_guide<BATCH>, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, d);
-                if (mark(2 * nd + 2 * d + 1)) return SPK_ERR_HIP;
-            }
+            if (mark(2 * nd + 2 * d)) return SPK_ERR_HIP;
+            hipLaunchKernelGGL(k_guide<BATCH>, dim3(ls.grid_guide), dim3(kGuideBlock), 0, st, s, b.cfg, b, *guide, slot,
+                               oslot);
+            if (mark(2 * nd + 2 * d + 1)) return SPK_ERR_HIP;
         }
     }
-    hipLaunchKernelGGL(k_film<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+template <bool BATCH>
+static int launch_film(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st, int last_slot,
+                       const GuideNetDev* guide) {
+    hipLaunchKernelGGL(k_film<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, cfg.max_depth, last_slot);
     if (guide && guide->record) {
         hipLaunchKernelGGL(k_guide_finalize, dim3(ls.grid_stream), dim3(256), 0, st, *guide, (const float4*)b.path_L);
         hipLaunchKernelGGL(k_guide_finalize_mark, dim3(1), dim3(64), 0, st, *guide);
     }
     SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+// A classic pass: all of its camera paths at once, then maxDepth + 1 bounces over shrinking queues.
+template <int B, bool BATCH>
+static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                         hipStream_t st, hipEvent_t* ev_ext, int n_ev_ext, bool count, const GuideNetDev* guide) {
+    hipLaunchKernelGGL(k_pass_init<BATCH>, dim3(1), dim3(256), 0, st, b.cfg, b.ps, b.passes, b.num_passes);
+    hipLaunchKernelGGL(k_camera<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, 0);
+    const int nd = cfg.max_depth + 1;
+    for (int d = 0; d <= cfg.max_depth; d++) {
+        int rc = launch_stages<B, BATCH>(s, b, cfg, ls, st, d, d + 1, d, nd, d == cfg.max_depth, ev_ext, n_ev_ext, count,
+                                         guide);
+        if (rc) return rc;
+    }
+    return launch_film<BATCH>(b, cfg, ls, st, 0, guide);
+}
+
+template <int B, bool BATCH>
+static int launch_iters_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                          hipStream_t st, int it0, int n_it, hipEvent_t* ev, int n_ev, bool count,
+                          const GuideNetDev* guide) {
+    for (int k = 0; k < n_it; k++) {
+        const int slot = (it0 + k) & 1;
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, b.cfg, b.ps, slot);
+        hipLaunchKernelGGL(k_camera<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, slot);
+        int rc = launch_stages<B, BATCH>(s, b, cfg, ls, st, slot, slot ^ 1, 0, 1, false, ev ? ev + 8 * k : nullptr,
+                                         ev ? 8 : 0, count, guide);
+        if (rc) return rc;
+    }
     return SPK_OK;
 }
 
@@ -2237,6 +2332,37 @@ int launch_pass(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, co
     case 256: return launch_pass_b<256, false>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
     default: return launch_pass_b<128, false>(s, b, cfg, ls, st, ev_ext, n_ev_ext, count, guide);
     }
+}
+
+int launch_regen_begin(const RenderBufs& b, bool batch, hipStream_t st) {
+    if (batch) hipLaunchKernelGGL(k_pass_init<true>, dim3(1), dim3(256), 0, st, b.cfg, b.ps, b.passes, b.num_passes);
+    else hipLaunchKernelGGL(k_pass_init<false>, dim3(1), dim3(256), 0, st, b.cfg, b.ps, b.passes, b.num_passes);
+    SPK_CHECK_HIP(hipGetLastError());
+    return SPK_OK;
+}
+
+int launch_regen_iters(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
+                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, bool count,
+                       const GuideNetDev* guide) {
+    const int n_ev = ev ? 8 * n_it : 0;
+    if (batch) {
+        switch (ls.block) {
+        case 64: return launch_iters_b<64, true>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+        case 256: return launch_iters_b<256, true>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+        default: return launch_iters_b<128, true>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+        }
+    }
+    switch (ls.block) {
+    case 64: return launch_iters_b<64, false>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+    case 256: return launch_iters_b<256, false>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+    default: return launch_iters_b<128, false>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);
+    }
+}
+
+int launch_regen_end(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st, bool batch,
+                     int last_slot, const GuideNetDev* guide) {
+    return batch ? launch_film<true>(b, cfg, ls, st, last_slot, guide)
+                 : launch_film<false>(b, cfg, ls, st, last_slot, guide);
 }
 
 int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st,

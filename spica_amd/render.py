@@ -80,7 +80,9 @@ class GpuScene:
 
 class GpuRender:
     def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False, count_visits=False,
-                 block_size=0):
+                 block_size=0, regen=True):
+        """queue_size: paths in flight; regen: passes refill their queues with new camera paths as
+        paths end (False: classic passes, all paths at once, queues shrinking by bounce)."""
         self.lib = gscene.lib
         self.gscene = gscene
         self.camera = camera
@@ -92,6 +94,7 @@ class GpuRender:
         self._opts.profile = 1 if profile else 0
         self._opts.use_graph = 1 if use_graph else 0
         self._opts.count_visits = 1 if count_visits else 0
+        self._opts.classic_passes = 0 if regen else 1
         h = ctypes.c_void_p()
         _abi.check(self.lib.spk_render_create(gscene.handle, ctypes.byref(self._cam), ctypes.byref(self._rd),
                                               ctypes.byref(self._opts), ctypes.byref(h)), "spk_render_create")
@@ -164,7 +167,7 @@ class Batch:
     """Many images (scene, camera, spp, seed each) rendered through shared wavefront passes."""
 
     def __init__(self, items, max_depth=16, rr_depth=3, sampler=_abi.SPK_SAMPLER_SOBOL, queue_size=0, block_size=0,
-                 graph=1):
+                 graph=1, regen=True):
         """items: list of (GpuScene, Camera, spp, seed)."""
         self.lib = _abi.load()
         self.items = list(items)
@@ -185,6 +188,7 @@ class Batch:
         o = _abi.SpkRenderOpts()
         o.queue_size = queue_size
         o.block_size = block_size
+        o.classic_passes = 0 if regen else 1
         h = ctypes.c_void_p()
         _abi.check(self.lib.spk_batch_create(ctypes.byref(d), ctypes.byref(o), ctypes.byref(h)), "spk_batch_create")
         self.handle = h

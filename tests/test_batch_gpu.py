@@ -35,13 +35,17 @@ def batch_setup(gpu):
         gs.close()
 
 
+@pytest.mark.parametrize("regen", [True, False])
 @pytest.mark.parametrize("graph", [0, 1, 2])
 @pytest.mark.parametrize("queue", [1 << 12, 1 << 16])
-def test_batch_equals_single_renders(batch_setup, graph, queue):
+def test_batch_equals_single_renders(batch_setup, graph, queue, regen):
+    """Also with regenerating passes (the queue keeps `queue` paths in flight and refills) and
+    classic ones (every path of a pass enters at once)."""
     from spica_amd import render
     wl, scenes, singles = batch_setup
     items = [(gs, cam, spp, seed) for gs, (_, cam, spp, seed) in zip(scenes, wl.items)]
-    b = render.Batch(items, max_depth=wl.max_depth, sampler=wl.sampler, queue_size=queue, block_size=64, graph=graph)
+    b = render.Batch(items, max_depth=wl.max_depth, sampler=wl.sampler, queue_size=queue, block_size=64, graph=graph,
+                     regen=regen)
     info = b.info()
     assert info["paths"] == wl.paths()
     for rep in range(2):  # a run starts from cleared films
@@ -53,7 +57,8 @@ def test_batch_equals_single_renders(batch_setup, graph, queue):
             assert np.array_equal(q, q_ref), (graph, queue, i)
     st = b.stats()
     assert st["paths"] == 2 * wl.paths()
-    print(f"graph {graph} queue {queue}: {info['passes']} passes, {info['chunks']} chunks, {info['paths']} paths")
+    print(f"graph {graph} queue {queue} regen {regen}: {info['passes']} passes, {info['chunks']} chunks, "
+          f"{info['paths']} paths")
     b.close()
 
 
@@ -64,15 +69,16 @@ def test_single_render_graph_replay_equals_launches(gpu):
     wl = configs.cornell_box(width=48, height=40, max_depth=8)
     gs = render.GpuScene(wl.scene)
     films = []
-    for use_graph in (False, True):
+    for use_graph, regen in ((False, False), (True, False), (False, True), (True, True)):
         gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=64), queue_size=1 << 13, block_size=64,
-                              use_graph=use_graph)
+                              use_graph=use_graph, regen=regen)
         gr.run(20)
         gr.run(33)
         films.append(gr.film())
         gr.close()
-    for a, b in zip(*films):
-        assert np.array_equal(a, b)
+    for f in films[1:]:
+        for a, b in zip(films[0], f):
+            assert np.array_equal(a, b)
     assert films[0][2].min() == 53
     gs.close()
 
