@@ -210,7 +210,7 @@ def guide_roofline(gs3, w3, queue, block, guide_cfg):
 
     from spica_amd import render
 
-    gr = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=96), queue_size=queue, block_size=block, profile=True)
+    gr = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=96), queue_size=queue, block_size=block, profile=2)
     g = render.Guide(**guide_cfg)
     gr.set_guide(g)
     gr.run(96, sync=True)
@@ -321,6 +321,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--queue", type=int, default=0, help="paths in flight (0 = autotune)")
     ap.add_argument("--block", type=int, default=0, help="traversal block size (0 = autotune)")
+    ap.add_argument("--batch-queue", type=int, default=0, help="paths in flight of the C2 batch (0 = autotune)")
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy-scene (C4) leg")
@@ -400,23 +401,32 @@ def main():
     ext_launches = st1["extend_launches"] - st0["extend_launches"]
     ext_rays = st1["extend_rays_timed"] - st0["extend_rays_timed"]
     value = rays / dt / 1e6
+    # The shading and occlusion kernels, from a render with events around every stage (after the
+    # timed region, which carries the closest-hit events only).
+    gp = render.GpuRender(gs, w.camera, w.render_desc(spp=args.warmup + args.steps), queue_size=args.queue,
+                          block_size=args.block, profile=2)
+    gp.run(args.warmup)
+    sp0 = gp.stats()
+    gp.run(args.steps, sync=True)
+    sp1 = gp.stats()
+    gp.close()
     # shading: per hit the queue entry (48 B) and hit (16 B) in, the triangle's shading record
     # (normal | material 16 B, ids 8 B), the path radiance read and written (32 B); per appended
     # shadow, MIS or continuation ray 48 B out.  Materials and lights stay in cache.
-    sh_ms = st1["shade_ms"] - st0["shade_ms"]
-    sh_hits = ext_rays
-    sh_bytes = 120 * sh_hits + 48 * ((st1["rays_shadow"] - st0["rays_shadow"]) + (st1["rays_mis"] - st0["rays_mis"]) +
-                                     (st1["rays_extend"] - st0["rays_extend"]) - paths)
-    so_ms = st1["shadow_ms"] - st0["shadow_ms"]
-    so_rays = st1["shadow_rays_timed"] - st0["shadow_rays_timed"]
+    sh_ms = sp1["shade_ms"] - sp0["shade_ms"]
+    sh_hits = sp1["extend_rays_timed"] - sp0["extend_rays_timed"]
+    sh_bytes = 120 * sh_hits + 48 * ((sp1["rays_shadow"] - sp0["rays_shadow"]) + (sp1["rays_mis"] - sp0["rays_mis"]) +
+                                     (sp1["rays_extend"] - sp0["rays_extend"]) - (sp1["paths"] - sp0["paths"]))
+    so_ms = sp1["shadow_ms"] - sp0["shadow_ms"]
+    so_rays = sp1["shadow_rays_timed"] - sp0["shadow_rays_timed"]
     other_roofs = {
         "k_shade": dict(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, bytes_per_hit=round(sh_bytes / max(1, sh_hits), 1),
-                        avg_launch_us=round(sh_ms * 1e3 / max(1, st1["shade_launches"] - st0["shade_launches"]), 2),
+                        avg_launch_us=round(sh_ms * 1e3 / max(1, sp1["shade_launches"] - sp0["shade_launches"]), 2),
                         achieved=round(sh_bytes / (sh_ms / 1e3) / 1e9, 2),
                         frac=round(sh_bytes / (sh_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)),
         "k_shadow": dict(bound="hbm", unit="GB/s", peak=HBM_PEAK_GBS, b_ray=round(b_shadow, 1),
                          v_node=round(v_node_s, 3), v_tri=round(v_tri_s, 3),
-                         avg_launch_us=round(so_ms * 1e3 / max(1, st1["shadow_launches"] - st0["shadow_launches"]), 2),
+                         avg_launch_us=round(so_ms * 1e3 / max(1, sp1["shadow_launches"] - sp0["shadow_launches"]), 2),
                          achieved=round(b_shadow * so_rays / (so_ms / 1e3) / 1e9, 2),
                          frac=round(b_shadow * so_rays / (so_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)),
     }
@@ -454,8 +464,13 @@ def main():
         bvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3  # summed over builds running side by side on 4 threads
         setup_s = gen_s + scene_s
         items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
-        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
-                         block_size=args.block, graph=1)
+        bq, btable = args.batch_queue, []
+        if bq == 0:
+            from spica_amd import autotune
+
+            bq, btable = autotune.tune_batch(items, wb.max_depth, wb.sampler, args.block, log=log)
+        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=bq, block_size=args.block,
+                         graph=1)
         st0 = b.stats()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -466,12 +481,12 @@ def main():
         brays = sum(st1[k] - st0[k] for k in ("rays_extend", "rays_shadow", "rays_mis"))
         info = b.info()
         batch = dict(workload="C2: 64 images, image i = C1 generator seed i with U{16..2048} spheres, W,H ~ U{128..1024}, "
-                              "spp ~ U{16..512}, maxDepth 16, Sobol seed i; one per-pass hipGraph replayed over shared passes",
+                              "spp ~ U{16..512}, maxDepth 16, Sobol seed i; regenerating passes, one captured group of iterations replayed",
                      images=len(items), triangles=int(tris), paths=int(info["paths"]), passes=int(info["passes"]),
                      seconds=round(bdt, 3), mrays_per_s=round(brays / bdt / 1e6, 3),
                      mpaths_per_s=round(info["paths"] / bdt / 1e6, 3), setup_s=round(setup_s, 1),
                      generate_s=round(gen_s, 1), scenes_s=round(scene_s, 1), bvh_build_thread_s=round(bvh_s, 1),
-                     queue=int(info["queue"]), block=args.block)
+                     queue=int(info["queue"]), block=args.block, autotune=btable)
         c2_ref = load_batch_reference()
         if c2_ref is not None:
             import numpy as np
@@ -493,8 +508,8 @@ def main():
         lscene_s = time.perf_counter() - t0
         lbvh_s = sum(g.stats()["build_ms"] for g in scenes) / 1e3
         items = [(g, cam, spp, seed) for g, (_, cam, spp, seed) in zip(scenes, wb.items)]
-        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=args.queue,
-                         block_size=args.block, graph=1)
+        b = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=bq, block_size=args.block,
+                         graph=1)
         st0 = b.stats()
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -97,9 +97,10 @@ typedef struct spk_render_opts {
     int64_t queue_size;     /* paths in flight per pass (0 = auto: 2^21) */
     int32_t block_size;     /* 64, 128 or 256 (0 = auto) */
     int32_t use_graph;      /* capture each pass into a hipGraph and replay it */
-    int32_t profile;        /* time every traversal launch with HIP events */
+    int32_t profile;        /* HIP events around every closest-hit launch (1), and also around
+                             * every guide, shading and occlusion launch (2) */
     int32_t count_visits;   /* count node fetches / triangle tests of the closest-hit stage */
-    int32_t classic_passes; /* 0 (default): a pass covers up to 2^26 paths and keeps queue_size of
+    int32_t classic_passes; /* 0 (default): a pass covers up to 2^28 paths and keeps queue_size of
                              * them in flight, replacing each finished path by the pass's next camera
                              * path, so every launch sees a full queue; 1: all of a pass's paths
                              * enter at once and the queues shrink bounce by bounce */
@@ -146,21 +147,21 @@ typedef struct spk_render_stats {
     int64_t rays_shadow;    /* NEE light-sample occlusion rays */
     int64_t rays_mis;       /* NEE BSDF-sample closest-hit rays */
     int64_t passes;
-    double extend_ms;       /* summed closest-hit traversal kernel time (profile = 1) */
+    double extend_ms;       /* summed closest-hit traversal kernel time (profile >= 1) */
     int64_t extend_launches;
     int64_t extend_rays_timed;
-    double shadow_ms;
+    double shadow_ms;       /* profile = 2: summed occlusion kernel time */
     int64_t shadow_launches;
-    double shade_ms;
+    double shade_ms;        /* profile = 2: summed shading kernel time */
     int64_t count_nodes;    /* count_visits = 1: node fetches of the closest-hit stage */
     int64_t count_tris;     /* count_visits = 1: triangle tests of the closest-hit stage */
     int64_t count_rays;     /* count_visits = 1: closest-hit rays those counts cover */
     int64_t guided_vertices;/* continuations sampled by the learned proposal (guided renders) */
     int64_t train_passes;   /* sample passes that recorded training data (guided renders) */
-    double guide_ms;        /* profile = 1, guided renders: summed guide-kernel (inference + sampling) time */
+    double guide_ms;        /* profile = 2, guided renders: summed guide-kernel (inference + sampling) time */
     int64_t guide_launches;
-    int64_t shade_launches; /* profile = 1: shading launches timed in shade_ms */
-    int64_t shadow_rays_timed; /* profile = 1: occlusion rays of the timed shadow launches */
+    int64_t shade_launches; /* profile = 2: shading launches timed in shade_ms */
+    int64_t shadow_rays_timed; /* profile >= 1: occlusion rays of the profiled runs */
     int64_t count_nodes_shadow; /* count_visits = 1: node fetches of the occlusion stage */
     int64_t count_tris_shadow;
     int64_t count_rays_shadow;

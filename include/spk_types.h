@@ -96,6 +96,8 @@ enum spk_bvh_builder {
 typedef struct spk_scene_build_opts {
     int32_t builder;             /* enum spk_bvh_builder */
     int32_t max_leaf;            /* triangles per leaf, 1..8 (spica: 4) */
+    int32_t treelet_passes;      /* LBVH: treelet-restructuring sweeps, 0..16 (default 6; 0 = the
+                                  * Morton tree as built) */
 } spk_scene_build_opts;
 
 /* Perspective camera (cameras/perspective.cc:23-74): "toWorld", "fov" (degrees,

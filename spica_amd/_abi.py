@@ -102,7 +102,7 @@ class SpkRenderDesc(ctypes.Structure):
 
 
 class SpkSceneBuildOpts(ctypes.Structure):
-    _fields_ = [("builder", ctypes.c_int32), ("max_leaf", ctypes.c_int32)]
+    _fields_ = [("builder", ctypes.c_int32), ("max_leaf", ctypes.c_int32), ("treelet_passes", ctypes.c_int32)]
 
 
 class SpkSceneStats(ctypes.Structure):

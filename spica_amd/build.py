@@ -22,6 +22,14 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared", "-std=c++17", "-Wa
 
 OBJ_DIR = os.path.join(HERE, "_build")
 
+# The stage kernels divide with a reciprocal and a multiply (within ~2 ulp of the correctly
+# rounded quotient, against ~10 instructions for it) and flush single-precision denormals,
+# which lets that reciprocal go without range scaling: shading and the learned proposal are
+# bound by their arithmetic (the static VALU count of k_shade and k_guide drops by a third).
+# The traversal's own divisions (inverse ray direction, triangle determinant) stay correctly
+# rounded by going through double precision.
+EXTRA_FLAGS = {"spk_kernels.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]}
+
 
 def build_lib(verbose=False, jobs=8):
     """Compiles every source to an object (in parallel; always, so a header change is never
@@ -33,7 +41,7 @@ def build_lib(verbose=False, jobs=8):
 
     def compile_one(src):
         obj = os.path.join(OBJ_DIR, src + ".o")
-        cmd = [HIPCC] + compile_flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + compile_flags + EXTRA_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -388,8 +388,10 @@ int collapse_bvh4(const std::vector<BvhNodeHost>& n2, std::vector<Bvh4NodeHost>&
 // scale = (hi - lo) / 255, raised (by doubling increments) until code 255 reaches hi.  A child's lower plane takes the
 // largest code whose decoding fmaf(q, scale, lo) is <= the plane, its upper plane the
 // smallest code whose decoding is >= it.  Device and host fmaf are both correctly rounded,
-// so the traversal decodes exactly the boxes checked here.  A flat axis has scale 0 and
-// decodes every code to the plane itself.
+// and decode_q flushes subnormals as the traversal kernels do, so the traversal decodes
+// exactly the boxes checked here (a negative subnormal lower bound is stored as -FLT_MIN,
+// which the flush keeps below the plane).  A flat axis has scale 0 and decodes every code
+// to the plane itself.
 void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHost>& out) {
     out.resize(in.size());
     for (size_t i = 0; i < in.size(); i++) {
@@ -405,10 +407,11 @@ void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHos
                     hi = std::max(hi, h.hi[a][c]);
                 }
             if (lo > hi) lo = hi = 0.0f;  // no children (never for a built tree)
+            if (flush_subnormal(lo) > lo) lo = -FLT_MIN;
             float sc = (hi - lo) / 255.0f;
             // raise the step until code 255 reaches hi; the increment doubles, so a nearly flat
             // box far from the origin needs a few rounds, not one per ulp of the step
-            for (float add = std::nextafter(sc, FLT_MAX) - sc; std::fmaf(255.0f, sc, lo) < hi; add *= 2.0f) sc += add;
+            for (float add = std::nextafter(sc, FLT_MAX) - sc; decode_q(255.0f, sc, lo) < hi; add *= 2.0f) sc += add;
             q.origin[a] = lo;
             q.scale[a] = sc;
             uint32_t wlo = 0, whi = 0;
@@ -419,10 +422,10 @@ void quantize_bvh4(const std::vector<Bvh4NodeHost>& in, std::vector<Bvh4QNodeHos
                     if (sc > 0.0f) {
                         ql = std::min(255, std::max(0, (int)std::floor((cl - lo) / sc)));
                         qh = std::min(255, std::max(0, (int)std::ceil((ch - lo) / sc)));
-                        while (ql > 0 && std::fmaf((float)ql, sc, lo) > cl) ql--;
-                        while (ql < 255 && std::fmaf((float)(ql + 1), sc, lo) <= cl) ql++;
-                        while (qh < 255 && std::fmaf((float)qh, sc, lo) < ch) qh++;
-                        while (qh > 0 && std::fmaf((float)(qh - 1), sc, lo) >= ch) qh--;
+                        while (ql > 0 && decode_q((float)ql, sc, lo) > cl) ql--;
+                        while (ql < 255 && decode_q((float)(ql + 1), sc, lo) <= cl) ql++;
+                        while (qh < 255 && decode_q((float)qh, sc, lo) < ch) qh++;
+                        while (qh > 0 && decode_q((float)(qh - 1), sc, lo) >= ch) qh--;
                     }
                 }
                 wlo |= (uint32_t)ql << (8 * c);

@@ -12,7 +12,13 @@
 //      SAH cost; a subtree of at most max_leaf triangles whose cost as one leaf is no
 //      higher than as a subtree is marked to collapse (the rule of Karras and Aila, "Fast
 //      parallel construction of high-quality BVHs", HPG 2013);
-//   4. emit binary nodes (collapsed subtrees as leaves over their contiguous ranges), then
+//   4. treelet restructuring (Karras and Aila 2013): bottom-up sweeps in which every node
+//      of at least kTreeletMinTris triangles takes the treelet of its up to 7 largest
+//      descendants and rebuilds the treelet's inner topology with the cheapest SAH cost
+//      (spk_treelet.h), followed by a top-down pass, one level per launch, that gives every
+//      subtree its contiguous range of the final triangle order (depth first);
+//   5. boxes, SAH costs and the leaf collapse of step 3 again over the restructured tree;
+//   6. emit binary nodes (collapsed subtrees as leaves over their contiguous ranges), then
 //      renumber the reachable nodes depth first on the host; the scene then collapses
 //      them into 4-wide nodes like the SAH tree's.
 // Node 0 is the root, as the traversal expects.  The tree answers the same closest hits
@@ -26,6 +32,7 @@
 #include <vector>
 
 #include "spk_internal.h"
+#include "spk_treelet.h"
 
 namespace spk {
 
@@ -146,6 +153,191 @@ __global__ void k_refit(const float* __restrict__ v, const uint32_t* __restrict_
     }
 }
 
+// Leaf k (node (n - 1) + k, triangle order[k] of the Morton order): box, SAH cost, one
+// triangle, and its position in the triangle order (the Morton position until restructured).
+__global__ void k_leaves(const float* __restrict__ v, const uint32_t* __restrict__ order, int n, BoxF* box, float* cost,
+                         int* size, int2* range, float ct) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const float* t = v + 9 * (size_t)order[k];
+        BoxF b;
+        for (int a = 0; a < 3; a++) {
+            b.lo[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+            b.hi[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+        }
+        const int node = (n - 1) + k;
+        box[node] = b;
+        cost[node] = ct * half_area(b);
+        size[node] = 1;
+        range[node] = make_int2(k, k);
+    }
+}
+
+constexpr int kTreeletBlock = 64;
+constexpr int kTreeletMinTris = 8;
+
+__device__ __forceinline__ TreeletBox to_tb(const BoxF& b) {
+    TreeletBox t;
+    for (int a = 0; a < 3; a++) {
+        t.lo[a] = b.lo[a];
+        t.hi[a] = b.hi[a];
+    }
+    return t;
+}
+
+// Re-optimises the treelet under p (p's box, size and cost are current, as are those of every
+// node below it).  Its leaves: p's children, then repeatedly the largest-area interior leaf
+// replaced by its two children, up to kTreeletLeaves; its interior nodes are p and the nodes
+// so replaced, and they are reused for the new topology.
+template <class Cost>
+__device__ void treelet_optimise(int p, int n, int2* child, int* parent, BoxF* box, float* cost, int* size, float cn,
+                                 Cost C) {
+    int leaf[kTreeletLeaves], inner[kTreeletLeaves - 1];
+    int nl = 2, ni = 1;
+    inner[0] = p;
+    const int2 c0 = child[p];
+    leaf[0] = c0.x;
+    leaf[1] = c0.y;
+    while (nl < kTreeletLeaves) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < nl; i++)
+            if (leaf[i] < n - 1) {
+                const float a = half_area(box[leaf[i]]);
+                if (a > ba) {
+                    ba = a;
+                    best = i;
+                }
+            }
+        if (best < 0) break;
+        const int x = leaf[best];
+        inner[ni++] = x;
+        const int2 cx = child[x];
+        leaf[best] = cx.x;
+        leaf[nl++] = cx.y;
+    }
+    if (nl < 3) return;  // two leaves have one topology
+    TreeletBox tb[kTreeletLeaves];
+    float lc[kTreeletLeaves];
+    for (int i = 0; i < nl; i++) {
+        tb[i] = to_tb(box[leaf[i]]);
+        lc[i] = cost[leaf[i]];
+    }
+    treelet_costs(tb, lc, nl, cn, C);
+    const uint32_t full = (1u << nl) - 1u;
+    if (!(C(full) < cost[p] * (1.0f - 1e-6f))) return;
+    // new topology top-down (each split's parts go to a leaf or the next spare interior node),
+    // then boxes, sizes and costs of the interior nodes, children before parents
+    uint32_t set[kTreeletLeaves - 1];
+    int node[kTreeletLeaves - 1];
+    int head = 0, tail = 1, next = 1;
+    set[0] = full;
+    node[0] = p;
+    while (head < tail) {
+        const uint32_t s = set[head];
+        const int x = node[head];
+        head++;
+        const uint32_t a = treelet_split(s, C);
+        const uint32_t parts[2] = {a, s ^ a};
+        int ids[2];
+        for (int k = 0; k < 2; k++) {
+            const uint32_t q = parts[k];
+            if ((q & (q - 1u)) == 0u) {
+                ids[k] = leaf[__ffs(q) - 1];
+            } else {
+                ids[k] = inner[next++];
+                set[tail] = q;
+                node[tail] = ids[k];
+                tail++;
+            }
+            parent[ids[k]] = x;
+        }
+        child[x] = make_int2(ids[0], ids[1]);
+    }
+    for (int j = tail - 1; j >= 0; j--) {
+        const int x = node[j];
+        const int2 ch = child[x];
+        const BoxF l = box[ch.x], r = box[ch.y];
+        BoxF m;
+        for (int a = 0; a < 3; a++) {
+            m.lo[a] = fminf(l.lo[a], r.lo[a]);
+            m.hi[a] = fmaxf(l.hi[a], r.hi[a]);
+        }
+        box[x] = m;
+        size[x] = size[ch.x] + size[ch.y];
+        cost[x] = cn * half_area(m) + cost[ch.x] + cost[ch.y];
+    }
+}
+
+// One bottom-up sweep: the second thread to reach a node (one atomic counter per node, as in
+// k_refit) has both subtrees final, refits the node and re-optimises its treelet.  Every
+// node a treelet touches lies under its root, whose subtrees are finished; the fence after
+// the counter makes the other subtree's writes visible.  Costs are the plain binary SAH
+// (the leaf collapse is decided after restructuring).
+__global__ __launch_bounds__(kTreeletBlock) void k_treelet(int n, int2* child, int* parent, BoxF* box, float* cost,
+                                                          int* size, uint32_t* flag, float cn) {
+    __shared__ float dp[kTreeletSubsets * kTreeletBlock];  // per thread one column of subset costs
+    float* col = dp + threadIdx.x;
+    auto C = [col](uint32_t s) -> float& { return col[s * kTreeletBlock]; };
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        int p = parent[(n - 1) + k];
+        while (p >= 0) {
+            if (atomicAdd(&flag[p], 1u) == 0u) break;
+            __threadfence();
+            const int2 ch = child[p];
+            const BoxF l = box[ch.x], r = box[ch.y];
+            BoxF m;
+            for (int a = 0; a < 3; a++) {
+                m.lo[a] = fminf(l.lo[a], r.lo[a]);
+                m.hi[a] = fmaxf(l.hi[a], r.hi[a]);
+            }
+            box[p] = m;
+            size[p] = size[ch.x] + size[ch.y];
+            cost[p] = cn * half_area(m) + cost[ch.x] + cost[ch.y];
+            if (size[p] >= kTreeletMinTris) treelet_optimise(p, n, child, parent, box, cost, size, cn, C);
+            __threadfence();
+            p = parent[p];
+        }
+    }
+}
+
+// One level of the top-down pass: each frontier node hands its range start to its children
+// (left first) and appends its interior children to the next frontier (one atomic per wave).
+__global__ void k_ranges_level(const int* __restrict__ frontier, uint32_t nf, const int2* __restrict__ child,
+                               const int* __restrict__ size, int2* range, int* next, uint32_t* next_count, int n) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nf; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        int c[2] = {-1, -1};
+        if (i < nf) {
+            const int x = frontier[i];
+            const int2 ch = child[x];
+            const int s = range[x].x;
+            const int sl = size[ch.x], sr = size[ch.y];
+            range[ch.x] = make_int2(s, s + sl - 1);
+            range[ch.y] = make_int2(s + sl, s + sl + sr - 1);
+            c[0] = ch.x < n - 1 ? ch.x : -1;
+            c[1] = ch.y < n - 1 ? ch.y : -1;
+        }
+        for (int k = 0; k < 2; k++) {
+            const bool push = c[k] >= 0;
+            const uint64_t m = __ballot(push);
+            if (m == 0) continue;
+            const int lane = __lane_id();
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t b = 0;
+            if (lane == leader) b = atomicAdd(next_count, (uint32_t)__popcll(m));
+            b = __shfl(b, leader);
+            if (push) next[b + __popcll(m & ((1ull << lane) - 1ull))] = c[k];
+        }
+    }
+}
+
+// Leaf k's triangle goes to its range start in the final order.
+__global__ void k_leaf_order(const uint32_t* __restrict__ morton_order, const int2* __restrict__ range, int n,
+                             uint32_t* order) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        order[range[(n - 1) + k].x] = morton_order[k];
+}
+
 __device__ __forceinline__ int32_t leaf_link_dev(uint32_t first, uint32_t count) {
     return ~(int32_t)((first << kLeafCountBits) | (count - 1));
 }
@@ -163,7 +355,7 @@ __global__ void k_emit(const int2* __restrict__ child, const int2* __restrict__ 
         for (int side = 0; side < 2; side++) {
             const int c = ids[side];
             if (c >= n - 1) {
-                h.d[side] = leaf_link_dev((uint32_t)(c - (n - 1)), 1u);
+                h.d[side] = leaf_link_dev((uint32_t)range[c].x, 1u);
             } else {
                 const int2 rg = range[c];
                 h.d[side] = collapse[c] ? leaf_link_dev((uint32_t)rg.x, (uint32_t)(rg.y - rg.x + 1)) : c;
@@ -175,10 +367,11 @@ __global__ void k_emit(const int2* __restrict__ child, const int2* __restrict__ 
 }
 
 struct Bufs {
-    DevBuf dv, codes, idx, codes_s, idx_s, tmp, child, range, parent, box, cost, collapse, flag, nodes;
+    DevBuf dv, codes, idx, codes_s, idx_s, tmp, child, range, parent, box, cost, collapse, flag, nodes, size, front[2],
+        front_n, order;
     ~Bufs() {
         for (DevBuf* b : {&dv, &codes, &idx, &codes_s, &idx_s, &tmp, &child, &range, &parent, &box, &cost, &collapse,
-                          &flag, &nodes})
+                          &flag, &nodes, &size, &front[0], &front[1], &front_n, &order})
             b->release();
     }
 };
@@ -238,7 +431,8 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
     int rc;
     if ((rc = B.dv.upload(verts, (size_t)n * 9 * 4)) || (rc = B.codes.alloc((size_t)n * 8)) ||
         (rc = B.idx.alloc((size_t)n * 4)) || (rc = B.codes_s.alloc((size_t)n * 8)) || (rc = B.idx_s.alloc((size_t)n * 4)) ||
-        (rc = B.child.alloc((size_t)n * 8)) || (rc = B.range.alloc((size_t)n * 8)) ||
+        (rc = B.child.alloc((size_t)n * 8)) || (rc = B.range.alloc((size_t)(2 * n) * 8)) ||
+        (rc = B.size.alloc((size_t)(2 * n) * 4)) ||
         (rc = B.parent.alloc((size_t)(2 * n) * 4)) || (rc = B.box.alloc((size_t)(2 * n) * sizeof(BoxF))) ||
         (rc = B.cost.alloc((size_t)(2 * n) * 4)) || (rc = B.collapse.alloc((size_t)n * 4)) ||
         (rc = B.flag.alloc((size_t)n * 4)) || (rc = B.nodes.alloc((size_t)(n - 1) * sizeof(BvhNodeHost))))
@@ -259,6 +453,46 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
     hipLaunchKernelGGL(k_hierarchy, dim3(blocks), dim3(256), 0, 0, B.codes_s.as<uint64_t>(), N, B.child.as<int2>(),
                        B.range.as<int2>(), B.parent.as<int>());
     SPK_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_leaves, dim3(blocks), dim3(256), 0, 0, B.dv.as<float>(), B.idx_s.as<uint32_t>(), N,
+                       B.box.as<BoxF>(), B.cost.as<float>(), B.size.as<int>(), B.range.as<int2>(), opts.tri_cost);
+    SPK_CHECK_HIP(hipGetLastError());
+    const uint32_t* leaf_order = B.idx_s.as<uint32_t>();
+    if (opts.treelet_passes > 0 && n >= kTreeletMinTris) {
+        const int tblocks = (int)std::min<int64_t>((n + kTreeletBlock - 1) / kTreeletBlock, 65536);
+        for (int pass = 0; pass < opts.treelet_passes; pass++) {
+            SPK_CHECK_HIP(hipMemset(B.flag.p, 0, (size_t)n * 4));
+            hipLaunchKernelGGL(k_treelet, dim3(tblocks), dim3(kTreeletBlock), 0, 0, N, B.child.as<int2>(),
+                               B.parent.as<int>(), B.box.as<BoxF>(), B.cost.as<float>(), B.size.as<int>(),
+                               B.flag.as<uint32_t>(), opts.node_cost);
+            SPK_CHECK_HIP(hipGetLastError());
+        }
+        // ranges top-down, one tree level per launch, from the root's [0, n - 1]
+        if ((rc = B.front[0].alloc((size_t)n * 4)) || (rc = B.front[1].alloc((size_t)n * 4)) ||
+            (rc = B.front_n.alloc(4)) || (rc = B.order.alloc((size_t)n * 4)))
+            return rc;
+        const int root = 0;
+        const int2 root_range = make_int2(0, N - 1);
+        SPK_CHECK_HIP(hipMemcpy(B.front[0].p, &root, 4, hipMemcpyHostToDevice));
+        SPK_CHECK_HIP(hipMemcpy(B.range.p, &root_range, 8, hipMemcpyHostToDevice));
+        uint32_t nf = 1;
+        for (int level = 0; nf > 0; level++) {
+            if (level > N) {
+                set_error("build_bvh_gpu: restructured tree deeper than its triangle count");
+                return SPK_ERR_STATE;
+            }
+            SPK_CHECK_HIP(hipMemset(B.front_n.p, 0, 4));
+            const int lb = (int)std::min<int64_t>((nf + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_ranges_level, dim3(lb), dim3(256), 0, 0, B.front[level & 1].as<int>(), nf,
+                               B.child.as<int2>(), B.size.as<int>(), B.range.as<int2>(), B.front[(level + 1) & 1].as<int>(),
+                               B.front_n.as<uint32_t>(), N);
+            SPK_CHECK_HIP(hipGetLastError());
+            SPK_CHECK_HIP(hipMemcpy(&nf, B.front_n.p, 4, hipMemcpyDeviceToHost));
+        }
+        hipLaunchKernelGGL(k_leaf_order, dim3(blocks), dim3(256), 0, 0, B.idx_s.as<uint32_t>(), B.range.as<int2>(), N,
+                           B.order.as<uint32_t>());
+        SPK_CHECK_HIP(hipGetLastError());
+        leaf_order = B.order.as<uint32_t>();
+    }
     SPK_CHECK_HIP(hipMemset(B.flag.p, 0, (size_t)n * 4));
     hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, 0, B.dv.as<float>(), B.idx_s.as<uint32_t>(), N,
                        B.child.as<int2>(), B.range.as<int2>(), B.parent.as<int>(), B.box.as<BoxF>(), B.cost.as<float>(),
@@ -270,7 +504,7 @@ int build_bvh_gpu(const float* verts, int64_t n, const BvhBuildOpts& opts, BvhRe
     std::vector<BvhNodeHost> raw((size_t)(n - 1));
     res.order.resize((size_t)n);
     SPK_CHECK_HIP(hipMemcpy(raw.data(), B.nodes.p, (size_t)(n - 1) * sizeof(BvhNodeHost), hipMemcpyDeviceToHost));
-    SPK_CHECK_HIP(hipMemcpy(res.order.data(), B.idx_s.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    SPK_CHECK_HIP(hipMemcpy(res.order.data(), leaf_order, (size_t)n * 4, hipMemcpyDeviceToHost));
     compact(raw, res);
     return SPK_OK;
 }

@@ -54,7 +54,7 @@ int zero_u32(uint32_t* p, int n, hipStream_t st);
 // Paths per regenerating pass (one path radiance slot each, 16 bytes): the in-flight queues are
 // the render's queue size, refilled from the pass's unstarted paths.
 #ifndef SPK_REGEN_PASS_LOG2
-#define SPK_REGEN_PASS_LOG2 26
+#define SPK_REGEN_PASS_LOG2 28
 #endif
 constexpr int64_t kRegenPassPaths = int64_t(1) << SPK_REGEN_PASS_LOG2;
 // Iterations of a regenerating pass launched between polls of its done flag.
@@ -335,6 +335,7 @@ int spk_scene_build_opts_default(spk_scene_build_opts* o) {
     }
     o->builder = SPK_BVH_SAH;
     o->max_leaf = 4;
+    o->treelet_passes = 6;
     return SPK_OK;
 }
 
@@ -347,9 +348,11 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     spk_scene_build_opts bop;
     spk_scene_build_opts_default(&bop);
     if (bopts) bop = *bopts;
-    if ((bop.builder != SPK_BVH_SAH && bop.builder != SPK_BVH_LBVH) || bop.max_leaf < 1 || bop.max_leaf > kMaxLeafTris) {
-        set_error("spk_scene_create: bad build options (builder %d, max_leaf %d; max_leaf in [1, %d])", bop.builder,
-                  bop.max_leaf, kMaxLeafTris);
+    if ((bop.builder != SPK_BVH_SAH && bop.builder != SPK_BVH_LBVH) || bop.max_leaf < 1 || bop.max_leaf > kMaxLeafTris ||
+        bop.treelet_passes < 0 || bop.treelet_passes > 16) {
+        set_error("spk_scene_create: bad build options (builder %d, max_leaf %d, treelet_passes %d; max_leaf in [1, %d], "
+                  "treelet_passes in [0, 16])",
+                  bop.builder, bop.max_leaf, bop.treelet_passes, kMaxLeafTris);
         return SPK_ERR_INVALID;
     }
     *out = nullptr;
@@ -423,6 +426,7 @@ int spk_scene_create_ex(const spk_scene_desc* d, const spk_scene_build_opts* bop
     auto t0 = std::chrono::steady_clock::now();
     BvhBuildOpts bo;
     bo.max_leaf = bop.max_leaf;
+    bo.treelet_passes = bop.treelet_passes;
     BvhResult bvh;
     if (bop.builder == SPK_BVH_LBVH) {
         if ((rc = build_bvh_gpu(d->verts, n, bo, bvh))) return rc;
@@ -819,8 +823,12 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     uint32_t qcap = regen ? (uint32_t)std::max<int64_t>(o.queue_size, kRegenPassPaths) : (uint32_t)o.queue_size;
     if (c.npix <= qcap) {
         // samples per pass; bounded so the per-path sample slot fits the queue flags
-        // (bits 7..31) and the film kernel's per-pixel loop stays short
+        // (bits 7..31) and the film kernel's per-pixel loop stays short.  A regenerating pass
+        // holds at most the render's planned samples (later runs just take more passes; how
+        // samples are cut into passes does not change the film), so small renders do not
+        // hold path slots for 2^28 paths.
         c.kmax = std::min<uint32_t>(qcap / c.npix, 1u << 14);
+        if (regen) c.kmax = std::min<uint32_t>(c.kmax, (uint32_t)std::max(1, desc->spp));
         c.chunks = 1;
         r->capacity = c.kmax * c.npix;
     } else {
@@ -992,7 +1000,7 @@ int run_pass_regen(spk_render* r, const SceneDev& sd, bool batch, bool use_graph
             if (!r->rg_exec) {
                 hipGraph_t gr;
                 SPK_CHECK_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
-                int crc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, 0, kRegenGroup, nullptr,
+                int crc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, 0, kRegenGroup, nullptr, 0,
                                              count, nullptr);
                 hipError_t e = hipStreamEndCapture(r->stream, &gr);
                 if (crc) return crc;
@@ -1009,8 +1017,8 @@ int run_pass_regen(spk_render* r, const SceneDev& sd, bool batch, bool use_graph
                 ev = &r->events[*ev_next];
                 *ev_next += 8 * kRegenGroup;
             }
-            if ((rc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, (int)it, kRegenGroup, ev, count,
-                                         net)))
+            if ((rc = launch_regen_iters(sd, r->bufs, r->cfg, r->shape, r->stream, batch, (int)it, kRegenGroup, ev,
+                                         r->opts.profile >= 2 ? 8 : 2, count, net)))
                 return rc;
         }
         it += kRegenGroup;
@@ -1026,7 +1034,8 @@ int run_pass_regen(spk_render* r, const SceneDev& sd, bool batch, bool use_graph
     return launch_regen_end(r->bufs, r->cfg, r->shape, r->stream, batch, (int)((it - 1) & 1), net);
 }
 
-// Adds the profile events of iterations [0, n_it) stored from index 0 (8 per iteration).
+// Adds the profile events of iterations [0, n_it) stored from index 0 (8 per iteration; the
+// closest-hit pair only at profile level 1).
 int regen_profile(spk_render* r, int64_t n_it, bool guided) {
     for (int64_t k = 0; k < n_it; k++) {
         float ms = 0.0f;
@@ -1034,6 +1043,7 @@ int regen_profile(spk_render* r, int64_t n_it, bool guided) {
         SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
         r->extend_ms += ms;
         r->extend_launches++;
+        if (r->opts.profile < 2) continue;
         if (guided) {
             SPK_CHECK_HIP(hipEventElapsedTime(&ms, e[2], e[3]));
             r->guide_ms += ms;
@@ -1103,7 +1113,9 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
     }
     for (int64_t p = 0; p < passes; p++) {
         hipEvent_t* ev = r->opts.profile ? &r->events[(size_t)p * nev] : nullptr;
-        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, ev ? nev : 0, r->opts.count_visits != 0,
+        // level 1 records the closest-hit pairs only (indices below 2 nd)
+        const int n_ev = !ev ? 0 : r->opts.profile >= 2 ? nev : 2 * nd;
+        if ((rc = launch_pass(r->scene->dev, r->bufs, c, r->shape, r->stream, ev, n_ev, r->opts.count_visits != 0,
                               net)))
             return rc;
     }
@@ -1119,6 +1131,7 @@ int run_passes(spk_render* r, int64_t first, int32_t n_spp, const GuideNetDev* n
                 SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[(size_t)p * nev + 2 * d], r->events[(size_t)p * nev + 2 * d + 1]));
                 r->extend_ms += ms;
                 r->extend_launches++;
+                if (r->opts.profile < 2) continue;
                 if (net && d < c.max_depth) {
                     const size_t g = (size_t)p * nev + 2 * (nd + d);
                     SPK_CHECK_HIP(hipEventElapsedTime(&ms, r->events[g], r->events[g + 1]));
@@ -1572,7 +1585,10 @@ int spk_batch_create(const spk_batch_desc* d, const spk_render_opts* opts, spk_b
     fc.to_world[0] = fc.to_world[5] = fc.to_world[10] = fc.to_world[15] = 1.0;
     spk_render_desc rd;
     memset(&rd, 0, sizeof(rd));
+    // the planned samples bound a regenerating pass: the largest image's, so passes hold as
+    // many whole samples of the film as fit
     rd.spp = 1;
+    for (int i = 0; i < d->num_images; i++) rd.spp = std::max(rd.spp, d->images[i].spp);
     rd.max_depth = d->max_depth;
     rd.rr_depth = d->rr_depth;
     rd.sampler = d->sampler;

@@ -71,14 +71,28 @@ __device__ __forceinline__ f32x4_t mfma_bf16(u32x4_t a, u32x4_t b, f32x4_t c) {
 }
 __device__ __forceinline__ u32x4_t ld16(const uint16_t* p) { return *reinterpret_cast<const u32x4_t*>(p); }
 
-// Bins 8g .. 8g+7 of the one-blob code of x: exp(-(32x - b - 1/2)^2 / 2).
+// Bins 8g .. 8g+7 of the one-blob code of x: exp(-(32x - b - 1/2)^2 / 2).  With
+// a = 32x - 8g - 1/2, bin j is e_j = exp(-(a - j)^2 / 2), and neighbouring bins differ by
+// e_{j+1} / e_j = exp(a - j - 1/2) and e_{j-1} / e_j = exp(j - a - 1/2): three exponentials
+// (the middle bin j = 4 and its two ratios) instead of eight, then ratios that shrink by e^-1
+// per step outwards.  What the underflow of e_4 loses is below exp(-42); |a| is clamped so the
+// ratios stay finite (every bin is zero long before).
 __device__ __forceinline__ u32x4_t blob8(float x, int g) {
-    const float a = (float)kGBins * x - 8.0f * (float)g - 0.5f;
+    const float a = fminf(fmaxf((float)kGBins * x - 8.0f * (float)g - 0.5f, -40.0f), 40.0f);
+    constexpr float kInvE = 0.36787944117144233f;
+    const float d = a - 4.0f;
     float v[8];
+    v[4] = __expf(-0.5f * d * d);
+    float up = __expf(d - 0.5f), dn = __expf(-d - 0.5f);
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        float d = a - (float)j;
-        v[j] = __expf(-0.5f * d * d);
+    for (int j = 5; j < 8; j++) {
+        v[j] = v[j - 1] * up;
+        up *= kInvE;
+    }
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {
+        v[j] = v[j + 1] * dn;
+        dn *= kInvE;
     }
     u32x4_t o;
     o[0] = pack_bf16(v[0], v[1]);
@@ -88,27 +102,34 @@ __device__ __forceinline__ u32x4_t blob8(float x, int g) {
     return o;
 }
 
-// Bias + ReLU on the C tiles of a 64-unit layer, packed as the next layer's two B chunks.
+// ReLU on the C tiles of a 64-unit layer (the bias is the accumulators' initial value),
+// packed as the next layer's two B chunks.
 template <int NCT>
-__device__ __forceinline__ void relu_pack(const f32x4_t (&acc)[4][NCT], const float* bias, int g, u32x4_t (&h)[2][NCT]) {
+__device__ __forceinline__ void relu_pack(const f32x4_t (&acc)[4][NCT], u32x4_t (&h)[2][NCT]) {
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        float b0[4], b1[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            b0[i] = bias[16 * (2 * c) + 4 * g + i];
-            b1[i] = bias[16 * (2 * c + 1) + 4 * g + i];
-        }
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) {
             const f32x4_t x = acc[2 * c][ct], y = acc[2 * c + 1][ct];
             u32x4_t o;
-            o[0] = pack_bf16(fmaxf(x[0] + b0[0], 0.0f), fmaxf(x[1] + b0[1], 0.0f));
-            o[1] = pack_bf16(fmaxf(x[2] + b0[2], 0.0f), fmaxf(x[3] + b0[3], 0.0f));
-            o[2] = pack_bf16(fmaxf(y[0] + b1[0], 0.0f), fmaxf(y[1] + b1[1], 0.0f));
-            o[3] = pack_bf16(fmaxf(y[2] + b1[2], 0.0f), fmaxf(y[3] + b1[3], 0.0f));
+            o[0] = pack_bf16(fmaxf(x[0], 0.0f), fmaxf(x[1], 0.0f));
+            o[1] = pack_bf16(fmaxf(x[2], 0.0f), fmaxf(x[3], 0.0f));
+            o[2] = pack_bf16(fmaxf(y[0], 0.0f), fmaxf(y[1], 0.0f));
+            o[3] = pack_bf16(fmaxf(y[2], 0.0f), fmaxf(y[3], 0.0f));
             h[c][ct] = o;
         }
+    }
+}
+
+// Accumulators of output tile mt start at the bias of units 16 mt + 4g .. 16 mt + 4g + 3.
+template <int NT, int NCT>
+__device__ __forceinline__ void bias_init(f32x4_t (&acc)[NT][NCT], const float* bias, int g) {
+#pragma unroll
+    for (int mt = 0; mt < NT; mt++) {
+        const f32x4_t b = f32x4_t{bias[16 * mt + 4 * g], bias[16 * mt + 4 * g + 1], bias[16 * mt + 4 * g + 2],
+                                  bias[16 * mt + 4 * g + 3]};
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = b;
     }
 }
 
@@ -126,10 +147,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
                                             f32x4_t (&out)[3][NCT], Hook& hook) {
     const int lane = __lane_id(), g = lane >> 4, r = lane & 15;
     f32x4_t acc[4][NCT];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    bias_init<4, NCT>(acc, bias, g);
 #pragma unroll
     for (int c = 0; c < kGIn; c++) {
         u32x4_t b[NCT];
@@ -146,15 +164,12 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
         }
     }
     u32x4_t h[2][NCT];
-    relu_pack<NCT>(acc, bias, g, h);
+    relu_pack<NCT>(acc, h);
     hook.template act<NCT>(1, h);
     const int offs[3] = {kGFwdW2, kGFwdW3, kGFwdW4};
 #pragma unroll
     for (int l = 0; l < 3; l++) {
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        bias_init<4, NCT>(acc, bias + 64 * (l + 1), g);
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -163,13 +178,10 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) acc[mt][ct] = mfma_bf16(a, h[c][ct], acc[mt][ct]);
             }
-        relu_pack<NCT>(acc, bias + 64 * (l + 1), g, h);
+        relu_pack<NCT>(acc, h);
         hook.template act<NCT>(l + 2, h);
     }
-#pragma unroll
-    for (int mt = 0; mt < 3; mt++)
-#pragma unroll
-        for (int ct = 0; ct < NCT; ct++) out[mt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    bias_init<3, NCT>(out, bias + 256, g);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -177,14 +189,6 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* __restrict__ w, cons
             const u32x4_t a = ld16(w + kGFwdW5 + (16 * mt + r) * kGHid + 32 * c + 8 * g);
 #pragma unroll
             for (int ct = 0; ct < NCT; ct++) out[mt][ct] = mfma_bf16(a, h[c][ct], out[mt][ct]);
-        }
-#pragma unroll
-    for (int mt = 0; mt < 3; mt++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float b = bias[256 + 16 * mt + 4 * g + i];
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) out[mt][ct][i] += b;
         }
 }
 

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cfloat>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -34,6 +36,7 @@ struct BvhBuildOpts {
     float tri_cost = 1.0f;            // SAH cost of one triangle test
     int64_t parallel_threshold = 65536;
     int parallel_depth = 6;
+    int treelet_passes = 0;           // LBVH: bottom-up treelet-restructuring sweeps
 };
 
 struct BvhNodeHost {
@@ -67,6 +70,13 @@ int collapse_bvh4(const std::vector<BvhNodeHost>& nodes2, std::vector<Bvh4NodeHo
 // hi x, hi y, hi z | links; byte c of a lo/hi word is child c's code q, decoded as
 // fmaf(q, scale, origin).  Codes are rounded outwards and checked against that exact
 // decoding, so every decoded box contains the child's box.
+//
+// The traversal kernels flush single-precision subnormals (inputs and results, sign kept),
+// so the host decodes the way they do.
+inline float flush_subnormal(float x) { return std::fabs(x) < FLT_MIN ? std::copysign(0.0f, x) : x; }
+inline float decode_q(float q, float scale, float origin) {
+    return flush_subnormal(std::fmaf(q, flush_subnormal(scale), flush_subnormal(origin)));
+}
 struct Bvh4QNodeHost {
     float origin[3];
     float scale[3];
@@ -181,7 +191,7 @@ int launch_batch_pass(const RenderBufs& b, const ConfigDev& cfg_host, const Laun
 // iteration it0 (queue slot it & 1; ev: 8 events per iteration or null), and the film.
 int launch_regen_begin(const RenderBufs& b, bool batch, hipStream_t st);
 int launch_regen_iters(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
-                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, bool count,
+                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, int ev_per_it, bool count,
                        const GuideNetDev* guide);
 int launch_regen_end(const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls, hipStream_t st, bool batch,
                      int last_slot, const GuideNetDev* guide);

@@ -132,6 +132,12 @@ __device__ __forceinline__ void block_append(const bool (&pred)[NQ], uint32_t* c
     }
 }
 
+// This file is compiled with fast single-precision division (reciprocal and multiply) and
+// flushes single-precision denormals; the two reciprocals of the traversal stay correctly
+// rounded: the double quotient of two floats rounded to float is the correctly rounded float
+// quotient (53 >= 2 * 24 + 2 bits make the double rounding innocuous).
+__device__ __forceinline__ float rcp_rn(float x) { return (float)(1.0 / (double)x); }
+
 // ---- ray/triangle (triangle.cc:98-178 in float) -----------------------------------------
 __device__ __forceinline__ bool tri_hit(const float4* __restrict__ tris, uint32_t k, float3 o, float3 d, float tmax,
                                         float& t, float& u, float& v) {
@@ -140,7 +146,7 @@ __device__ __forceinline__ bool tri_hit(const float4* __restrict__ tris, uint32_
     float3 pv = cross(d, e2);
     float det = dot(e1, pv);
     if (det > -kEPS && det < kEPS) return false;
-    float inv = 1.0f / det;
+    float inv = rcp_rn(det);
     float3 tv = sub(o, xyz(T0));
     float uu = dot(tv, pv) * inv;
     if (uu < 0.0f || uu > 1.0f) return false;
@@ -159,7 +165,7 @@ __device__ __forceinline__ float3 safe_inv(float3 d) {
     // ray.cc:43-47: zero components become INFTY; tiny ones are clamped to it
     auto one = [](float x) {
         if (x == 0.0f) return kINFTY;
-        float r = 1.0f / x;
+        float r = rcp_rn(x);
         return fabsf(r) > kINFTY ? copysignf(kINFTY, r) : r;
     };
     return mk(one(d.x), one(d.y), one(d.z));
@@ -1496,8 +1502,12 @@ __device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
 // the queue appends are aggregated per block (block_append).
 constexpr int kShadeBlock = 256;
 
+// The basic kernel is held to 128 VGPRs (4 waves per SIMD); the wide one holds more lobes.
+template <bool WIDE>
+constexpr int shade_waves() { return WIDE ? 1 : 4; }
+
 template <int BLOCK, bool GUIDED, bool BATCH, bool WIDE>
-__global__ __launch_bounds__(BLOCK) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(shade_waves<WIDE>()))) void k_shade(SceneDev sc_arg, const ConfigDev* __restrict__ cfgp, RenderBufs rb,
                                                  int slot, int oslot) {
     PassState* ps = rb.ps;
     const uint32_t count = ps->countA[slot];
@@ -2311,6 +2321,8 @@ static int launch_pass_b(const SceneDev& s, const RenderBufs& b, const ConfigDev
 }
 
 template <int B, bool BATCH>
+// n_ev: profile events recorded per iteration (8 = every stage, 2 = the closest hit), eight
+// slots apart.
 static int launch_iters_b(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
                           hipStream_t st, int it0, int n_it, hipEvent_t* ev, int n_ev, bool count,
                           const GuideNetDev* guide) {
@@ -2319,7 +2331,7 @@ static int launch_iters_b(const SceneDev& s, const RenderBufs& b, const ConfigDe
         hipLaunchKernelGGL(k_plan, dim3(1), dim3(64), 0, st, b.cfg, b.ps, slot);
         hipLaunchKernelGGL(k_camera<BATCH>, dim3(ls.grid_stream), dim3(256), 0, st, b.cfg, b, slot);
         int rc = launch_stages<B, BATCH>(s, b, cfg, ls, st, slot, slot ^ 1, 0, 1, false, ev ? ev + 8 * k : nullptr,
-                                         ev ? 8 : 0, count, guide);
+                                         n_ev, count, guide);
         if (rc) return rc;
     }
     return SPK_OK;
@@ -2342,9 +2354,9 @@ int launch_regen_begin(const RenderBufs& b, bool batch, hipStream_t st) {
 }
 
 int launch_regen_iters(const SceneDev& s, const RenderBufs& b, const ConfigDev& cfg, const LaunchShape& ls,
-                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, bool count,
+                       hipStream_t st, bool batch, int it0, int n_it, hipEvent_t* ev, int ev_per_it, bool count,
                        const GuideNetDev* guide) {
-    const int n_ev = ev ? 8 * n_it : 0;
+    const int n_ev = ev ? ev_per_it : 0;
     if (batch) {
         switch (ls.block) {
         case 64: return launch_iters_b<64, true>(s, b, cfg, ls, st, it0, n_it, ev, n_ev, count, guide);

@@ -24,9 +24,10 @@ def _ptr(t):
 
 
 class GpuScene:
-    """builder: "sah" (host binned SAH, spica's BVH) or "lbvh" (linear BVH built on the device)."""
+    """builder: "sah" (host binned SAH, spica's BVH) or "lbvh" (linear BVH built on the device);
+    treelet_passes: LBVH treelet-restructuring sweeps (None = the library default, 6)."""
 
-    def __init__(self, scene, device=0, builder="sah", max_leaf=4):
+    def __init__(self, scene, device=0, builder="sah", max_leaf=4, treelet_passes=None):
         self.lib = init(device)
         self.scene = scene
         self._desc = scene.desc()
@@ -34,6 +35,8 @@ class GpuScene:
         _abi.check(self.lib.spk_scene_build_opts_default(ctypes.byref(bo)), "spk_scene_build_opts_default")
         bo.builder = {"sah": _abi.SPK_BVH_SAH, "lbvh": _abi.SPK_BVH_LBVH}[builder]
         bo.max_leaf = max_leaf
+        if treelet_passes is not None:
+            bo.treelet_passes = int(treelet_passes)
         h = ctypes.c_void_p()
         _abi.check(self.lib.spk_scene_create_ex(ctypes.byref(self._desc), ctypes.byref(bo), ctypes.byref(h)),
                    "spk_scene_create_ex")
@@ -82,7 +85,8 @@ class GpuRender:
     def __init__(self, gscene, camera, render_desc, queue_size=0, profile=False, use_graph=False, count_visits=False,
                  block_size=0, regen=True):
         """queue_size: paths in flight; regen: passes refill their queues with new camera paths as
-        paths end (False: classic passes, all paths at once, queues shrinking by bounce)."""
+        paths end (False: classic passes, all paths at once, queues shrinking by bounce); profile:
+        HIP events around the closest-hit launches (1 or True) or around every stage (2)."""
         self.lib = gscene.lib
         self.gscene = gscene
         self.camera = camera
@@ -91,7 +95,7 @@ class GpuRender:
         self._opts = _abi.SpkRenderOpts()
         self._opts.queue_size = int(queue_size)
         self._opts.block_size = int(block_size)
-        self._opts.profile = 1 if profile else 0
+        self._opts.profile = int(profile) if profile else 0
         self._opts.use_graph = 1 if use_graph else 0
         self._opts.count_visits = 1 if count_visits else 0
         self._opts.classic_passes = 0 if regen else 1

@@ -3,7 +3,8 @@
 //   quant_check <triangles.bin>   build the SAH tree of the triangles (int64 n, then n x 9 float32
 //                                 vertices), collapse it to 4-wide nodes, quantise, and check that
 //                                 every decoded child box, decoded exactly as the traversal does
-//                                 (fmaf(q, scale, origin)), contains the child's float box and that
+//                                 (fmaf(q, scale, origin) with subnormals flushed), contains the
+//                                 child's float box and that
 //                                 the links are unchanged
 //   quant_check --edge            the same check on hand-made nodes: flat and nearly flat boxes
 //                                 (0..9 ulps wide) at offsets from 0 to 3e7, subnormal offsets and
@@ -27,8 +28,8 @@ static long check(const std::vector<Bvh4NodeHost>& w, const std::vector<Bvh4QNod
             if (w[i].link[c] == kBvh4Empty) continue;
             double vq = 1.0, vf = 1.0;
             for (int a = 0; a < 3; a++) {
-                const float lo = std::fmaf((float)((q[i].qlo[a] >> (8 * c)) & 255u), q[i].scale[a], q[i].origin[a]);
-                const float hi = std::fmaf((float)((q[i].qhi[a] >> (8 * c)) & 255u), q[i].scale[a], q[i].origin[a]);
+                const float lo = decode_q((float)((q[i].qlo[a] >> (8 * c)) & 255u), q[i].scale[a], q[i].origin[a]);
+                const float hi = decode_q((float)((q[i].qhi[a] >> (8 * c)) & 255u), q[i].scale[a], q[i].origin[a]);
                 if (!(lo <= w[i].lo[a][c]) || !(hi >= w[i].hi[a][c])) bad++;
                 vq *= std::fmax(1e-6, (double)hi - lo);
                 vf *= std::fmax(1e-6, (double)w[i].hi[a][c] - w[i].lo[a][c]);

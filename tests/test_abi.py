@@ -66,11 +66,12 @@ def test_scene_build_options_are_checked():
     lib = _abi.load()
     bo = _abi.SpkSceneBuildOpts()
     assert lib.spk_scene_build_opts_default(ctypes.byref(bo)) == _abi.SPK_OK
-    assert (bo.builder, bo.max_leaf) == (_abi.SPK_BVH_SAH, 4)
+    assert (bo.builder, bo.max_leaf, bo.treelet_passes) == (_abi.SPK_BVH_SAH, 4, 6)
     d = configs.cornell_box().scene.desc()
     h = ctypes.c_void_p()
-    for builder, leaf in ((2, 4), (-1, 4), (_abi.SPK_BVH_LBVH, 0), (_abi.SPK_BVH_LBVH, 9), (_abi.SPK_BVH_SAH, 100)):
-        bo.builder, bo.max_leaf = builder, leaf
+    for builder, leaf, tp in ((2, 4, 6), (-1, 4, 6), (_abi.SPK_BVH_LBVH, 0, 6), (_abi.SPK_BVH_LBVH, 9, 6),
+                              (_abi.SPK_BVH_SAH, 100, 6), (_abi.SPK_BVH_LBVH, 4, -1), (_abi.SPK_BVH_LBVH, 4, 17)):
+        bo.builder, bo.max_leaf, bo.treelet_passes = builder, leaf, tp
         rc = lib.spk_scene_create_ex(ctypes.byref(d), ctypes.byref(bo), ctypes.byref(h))
         assert rc == _abi.SPK_ERR_INVALID and b"build options" in lib.spk_last_error()
     assert lib.spk_scene_build_opts_default(None) == _abi.SPK_ERR_INVALID

@@ -171,16 +171,19 @@ def test_guided_render_is_unbiased(gpu):
 
 def test_guided_render_reduces_variance(gpu):
     """On the occluded-glossy scene (light inside a box, open towards the ceiling) the
-    trained proposal lowers the per-sample variance after training, at equal samples."""
+    trained proposal lowers the per-sample variance after training, at equal samples.
+    Online training is noisy (one outlier record can steer an Adam step), so three guide
+    seeds are trained: the median gain must exceed 1.5 and no guide may be worse than plain
+    sampling (five seeds at 640x360 ranged 1.35-2.70, typically ~2.5)."""
     from spica_amd import configs, render
     wl = configs.c3_occluded_glossy(width=640, height=360)
     gs = render.GpuScene(wl.scene)
-    out = {}
-    for guided in (False, True):
+
+    def relvar(seed):
         gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=512), queue_size=1 << 22, block_size=64)
         g = None
-        if guided:
-            g = render.Guide(batch_size=1 << 17, train_spp=64, train_pass_spp=4, steps_per_pass=2)
+        if seed is not None:
+            g = render.Guide(batch_size=1 << 17, train_spp=64, train_pass_spp=4, steps_per_pass=2, seed=seed)
             gr.set_guide(g)
         gr.run(64)
         s0, q0, c0 = gr.film()
@@ -189,14 +192,18 @@ def test_guided_render_reduces_variance(gpu):
         n = (c1 - c0)[..., None]
         m = (s1 - s0) / n
         v = np.maximum((q1 - q0) / n - m * m, 0.0)
-        out[guided] = float((v / (m * m + 1e-2)).mean())
         gr.close()
         if g:
             g.close()
-    print(f"relative variance per sample after training: plain {out[False]:.4f}, guided {out[True]:.4f}, "
-          f"ratio {out[False] / out[True]:.2f}")
+        return float((v / (m * m + 1e-2)).mean())
+
+    plain = relvar(None)
+    ratios = [plain / relvar(seed) for seed in (1, 2, 3)]
+    print(f"relative variance per sample after training: plain {plain:.4f}, gain per guide seed "
+          f"{', '.join(f'{r:.2f}' for r in ratios)}")
     # about 1 training record per pixel and round: the gain grows with the frame (3x at 1280x720)
-    assert out[False] / out[True] > 1.5
+    assert float(np.median(ratios)) > 1.5
+    assert min(ratios) > 1.0
     gs.close()
 
 
