@@ -326,7 +326,8 @@ struct GuideState {
     uint32_t perm_off;
     float loss_sum;        // sum over the batch of -(F/p) log q
     float weight_sum;
-    uint32_t pad2[2];
+    float gscale;          // this step's gradient scale (norm clipping)
+    float gnorm_ema;       // running mean of the clipped gradient norms (0 before the first step)
 };
 
 }  // namespace spk

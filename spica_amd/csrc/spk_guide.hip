@@ -457,6 +457,42 @@ __global__ __launch_bounds__(64) void k_guide_dw(TrainBufs tb, const GuideState*
     }
 }
 
+// Gradient-norm clipping of a training step: one block sums the squared mean gradient over
+// all parameters; a step whose norm exceeds kGuideClip times the running mean of earlier
+// (clipped) norms is scaled back to that bound.  A record whose radiance estimate F/p is far
+// above the rest can otherwise steer a whole Adam step.  The first step is never clipped.
+#ifndef SPK_GUIDE_CLIP
+#define SPK_GUIDE_CLIP 0
+#endif
+constexpr float kGuideClip = SPK_GUIDE_CLIP;
+constexpr int kGnormBlock = 1024;
+
+__global__ __launch_bounds__(kGnormBlock) void k_guide_gnorm(const float* __restrict__ partial, int splits,
+                                                             GuideState* gs) {
+    __shared__ float red[kGnormBlock / 64];
+    if (!gs->active) return;
+    const float inv_b = 1.0f / (float)gs->b_eff;
+    float acc = 0.0f;
+    for (int p = threadIdx.x; p < kGParams; p += kGnormBlock) {
+        float s = 0.0f;
+        for (int k = 0; k < splits; k++) s += partial[(size_t)k * kGParams + p];
+        s *= inv_b;
+        acc += s * s;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int i = 0; i < kGnormBlock / 64; i++) t += red[i];
+        const float norm = sqrtf(t), ema = gs->gnorm_ema;
+        const float bound = ema > 0.0f ? kGuideClip * ema : 3.0e38f;
+        const float kept = fminf(norm, bound);
+        gs->gscale = norm > bound ? bound / norm : 1.0f;
+        gs->gnorm_ema = ema > 0.0f ? 0.9f * ema + 0.1f * kept : norm;
+    }
+}
+
 __global__ void k_guide_adam(float* params, float* m, float* v, const float* __restrict__ partial, int splits,
                              const GuideState* __restrict__ gs, AdamCfg cfg, float* grad_out, uint16_t* wf, uint16_t* wb,
                              float* bias) {
@@ -464,11 +500,12 @@ __global__ void k_guide_adam(float* params, float* m, float* v, const float* __r
     if (p >= kGParams || !gs->active) return;
     float s = 0.0f;
     for (int k = 0; k < splits; k++) s += partial[(size_t)k * kGParams + p];
-    const float gr = s / (float)gs->b_eff;
+    float gr = s / (float)gs->b_eff;
     if (grad_out) {
         grad_out[p] = gr;
         return;
     }
+    if (kGuideClip > 0.0f) gr *= gs->gscale;
     const float t = (float)gs->t;
     const float mm = cfg.beta1 * m[p] + (1.0f - cfg.beta1) * gr;
     const float vv = cfg.beta2 * v[p] + (1.0f - cfg.beta2) * gr * gr;
@@ -569,6 +606,8 @@ int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNe
     const uint32_t cps = kGuideColsPerSplit;
     const uint32_t splits = (tb.ncols + cps - 1) / cps;
     hipLaunchKernelGGL(k_guide_dw, dim3(8, splits), dim3(64), 0, st, tb, (const GuideState*)gb.gs, gb.partial, cps);
+    if (kGuideClip > 0.0f && !grad_out)
+        hipLaunchKernelGGL(k_guide_gnorm, dim3(1), dim3(kGnormBlock), 0, st, (const float*)gb.partial, (int)splits, gb.gs);
     hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gb.params, gb.m, gb.v,
                        (const float*)gb.partial, (int)splits, (const GuideState*)gb.gs, cfg, grad_out, gb.wf, gb.wb,
                        gb.bias);
