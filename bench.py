@@ -204,22 +204,26 @@ def time_to_relmse(gs, w, ref, guide_cfg, target, max_spp, queue, block, chunk=3
 
 def guide_roofline(gs3, w3, queue, block, guide_cfg):
     """The guide kernel (one-blob MLP inference on MFMA + mixture/BSDF sampling per vertex) and the
-    training step, against the bf16 MFMA peak: a short profiled guided render (64 training + 32
-    samples; HIP events around every guide launch) and timed training steps on one batch."""
+    training step, against the bf16 MFMA peak: a profiled guided render (HIP events around every
+    guide launch) over the 64 samples after the 64 training samples, the regime the rest of a
+    time-to-relMSE render runs in (training passes hold 4 samples each and drain often), and
+    timed training steps on one batch."""
     import numpy as np
 
     from spica_amd import render
 
-    gr = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=96), queue_size=queue, block_size=block, profile=2)
+    gr = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=128), queue_size=queue, block_size=block, profile=2)
     g = render.Guide(**guide_cfg)
     gr.set_guide(g)
-    gr.run(96, sync=True)
-    st = gr.stats()
+    gr.run(64, sync=True)
+    st0 = gr.stats()
+    gr.run(64, sync=True)
+    st1 = gr.stats()
     gr.close()
     g.close()
-    launches = max(1, st["guide_launches"])
-    verts = st["guided_vertices"] / launches
-    avg_s = st["guide_ms"] / 1e3 / launches
+    launches = max(1, st1["guide_launches"] - st0["guide_launches"])
+    verts = (st1["guided_vertices"] - st0["guided_vertices"]) / launches
+    avg_s = (st1["guide_ms"] - st0["guide_ms"]) / 1e3 / launches
     achieved = GUIDE_FWD_FLOP * verts / avg_s / 1e12
     # training: steps on one host batch of batch_size records; the first call includes the upload
     b = guide_cfg["batch_size"]
@@ -499,7 +503,28 @@ def main():
                                    relmse=float(f"{float(((m - refm) ** 2 / (refm ** 2 + 1e-2)).mean()):.4g}"))
             batch["relmse"] = dict(images=rel, eps=1e-2,
                                    reference=f"{c2_ref['spp']}-spp float64 CPU render, stratified pixels per image")
+        # the same batch as one whole-batch hipGraph launch: classic passes (a fixed launch sequence
+        # per pass, so the graph holds every pass), films checked against the regenerating run's
+        import numpy as np
+
+        films = [b.film(i) for i in (0, 1)]
         b.close()
+        bg = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=bq, block_size=args.block,
+                          graph=2, regen=False)
+        bg.run(sync=True)  # capture and first launch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bg.run(sync=True)
+        torch.cuda.synchronize()
+        gdt = time.perf_counter() - t0
+        same = all(np.array_equal(x, y) for i, f in zip((0, 1), films) for x, y in zip(bg.film(i), f))
+        ginfo = bg.info()
+        bg.close()
+        batch["whole_batch_graph"] = dict(seconds=round(gdt, 3), mrays_per_s=round(brays / gdt / 1e6, 3),
+                                          passes=int(ginfo["passes"]), queue=int(ginfo["queue"]),
+                                          films_equal_images_0_1=bool(same),
+                                          note="classic passes, the whole batch captured as one graph and launched once")
+        log(f"bench: C2 whole-batch graph {batch['whole_batch_graph']}")
         for g in scenes:
             g.close()
         # the same batch over trees built on the device: the time to the first pixel

@@ -77,7 +77,9 @@ int spk_scene_stats_get(const spk_scene* scene, spk_scene_stats* out);
  * (closest t, or NULL).  Directions are normalised first, as spica's Ray does
  * (sources/core/ray.cc:12-20).  Replaces Scene::intersect (sources/core/scene.cc:37-43)
  * -> BVHAccel::intersectBVH (sources/accelerators/bvh.cc:331-387).  `stream` is a
- * hipStream_t (NULL = default stream); the call is asynchronous on it. */
+ * hipStream_t (NULL = default stream); the call is asynchronous on it.  The scene keeps
+ * one traversal scratch area (work counter, stack spill): trace one scene on one stream
+ * at a time (calls on the same stream are ordered; calls on several streams race). */
 int spk_trace(spk_scene* scene, const float* org, const float* dir, const float* tmax, int64_t n, int32_t any_hit,
               int32_t* prim_out, float* t_out, void* stream);
 

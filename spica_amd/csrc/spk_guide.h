@@ -102,6 +102,11 @@ __device__ __forceinline__ u32x4_t blob8(float x, int g) {
     return o;
 }
 
+// ReLU as an integer max of the float's bits (a negative float has the sign bit set, so it
+// compares below 0 as an int; -0 becomes +0): one v_max_i32, where fmaxf (or a select the
+// compiler turns into one) adds a canonicalising max in front of every ReLU in IEEE mode.
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 // ReLU on the C tiles of a 64-unit layer (the bias is the accumulators' initial value),
 // packed as the next layer's two B chunks.
 template <int NCT>
@@ -112,10 +117,10 @@ __device__ __forceinline__ void relu_pack(const f32x4_t (&acc)[4][NCT], u32x4_t 
         for (int ct = 0; ct < NCT; ct++) {
             const f32x4_t x = acc[2 * c][ct], y = acc[2 * c + 1][ct];
             u32x4_t o;
-            o[0] = pack_bf16(fmaxf(x[0], 0.0f), fmaxf(x[1], 0.0f));
-            o[1] = pack_bf16(fmaxf(x[2], 0.0f), fmaxf(x[3], 0.0f));
-            o[2] = pack_bf16(fmaxf(y[0], 0.0f), fmaxf(y[1], 0.0f));
-            o[3] = pack_bf16(fmaxf(y[2], 0.0f), fmaxf(y[3], 0.0f));
+            o[0] = pack_bf16(relu(x[0]), relu(x[1]));
+            o[1] = pack_bf16(relu(x[2]), relu(x[3]));
+            o[2] = pack_bf16(relu(y[0]), relu(y[1]));
+            o[3] = pack_bf16(relu(y[2]), relu(y[3]));
             h[c][ct] = o;
         }
     }
@@ -223,28 +228,58 @@ struct GuideMix {
     float lognorm[kGLobes];                        // log(kappa / (2 pi (1 - e^{-2 kappa})))
 };
 
-// o: the 40 raw outputs of one query (lobe k at 5k .. 5k+4).
+// One lobe of the mixture from the raw outputs o (lobe k at 5k .. 5k+4) and the log of the
+// logits' softmax normaliser ls: unit mean, concentration, and log(pi_k) + log of the vMF
+// normaliser kappa / (2 pi (1 - e^{-2 kappa})).
+struct GuideLobe {
+    float mx, my, mz, kappa, logw;
+};
 template <class Out>
-__device__ __forceinline__ void mix_from_outputs(Out o, GuideMix& m) {
+__device__ __forceinline__ GuideLobe mix_lobe(Out o, int k, float ls) {
+    GuideLobe l;
+    const float x = o(5 * k + 1), y = o(5 * k + 2), z = o(5 * k + 3);
+    const float inv = rsqrtf(fmaxf(x * x + y * y + z * z, 1.0e-12f));
+    l.mx = x * inv;
+    l.my = y * inv;
+    l.mz = z * inv;
+    l.kappa = __expf(fminf(fmaxf(o(5 * k + 4), kGLogKappaLo), kGLogKappaHi));
+    // 1 - e^{-2 kappa} >= 0.036 for kappa >= e^-4: the fast exponential keeps ~6e-6 relative
+    // accuracy where expm1 would cost ~20 instructions
+    l.logw = o(5 * k) - ls + __logf(l.kappa) - 1.8378770664093453f /* log 2pi */ -
+             __logf(1.0f - __expf(-2.0f * l.kappa));
+    return l;
+}
+
+// log of the softmax normaliser of the 8 logits.
+template <class Out>
+__device__ __forceinline__ float mix_log_norm(Out o) {
     float amax = -3.0e38f;
 #pragma unroll
     for (int k = 0; k < kGLobes; k++) amax = fmaxf(amax, o(5 * k));
     float s = 0.0f;
 #pragma unroll
     for (int k = 0; k < kGLobes; k++) s += __expf(o(5 * k) - amax);
-    const float ls = __logf(s) + amax;
+    return __logf(s) + amax;
+}
+
+// o: the 40 raw outputs of one query (lobe k at 5k .. 5k+4).
+template <class Out>
+__device__ __forceinline__ void mix_from_outputs(Out o, GuideMix& m) {
+    const float ls = mix_log_norm(o);
 #pragma unroll
     for (int k = 0; k < kGLobes; k++) {
+        const GuideLobe l = mix_lobe(o, k, ls);
         m.logpi[k] = o(5 * k) - ls;
-        float x = o(5 * k + 1), y = o(5 * k + 2), z = o(5 * k + 3);
-        float inv = rsqrtf(fmaxf(x * x + y * y + z * z, 1.0e-12f));
-        m.mx[k] = x * inv;
-        m.my[k] = y * inv;
-        m.mz[k] = z * inv;
-        float kap = __expf(fminf(fmaxf(o(5 * k + 4), kGLogKappaLo), kGLogKappaHi));
-        m.kappa[k] = kap;
-        m.lognorm[k] = __logf(kap) - 1.8378770664093453f /* log 2pi */ - __logf(-expm1f(-2.0f * kap));
+        m.mx[k] = l.mx;
+        m.my[k] = l.my;
+        m.mz[k] = l.mz;
+        m.kappa[k] = l.kappa;
+        m.lognorm[k] = l.logw - m.logpi[k];
     }
+}
+
+__device__ __forceinline__ float lobe_log_term(const GuideLobe& l, float wx, float wy, float wz) {
+    return l.logw + l.kappa * (l.mx * wx + l.my * wy + l.mz * wz - 1.0f);
 }
 
 __device__ __forceinline__ float mix_log_pdf(const GuideMix& m, float wx, float wy, float wz) {
@@ -261,6 +296,21 @@ __device__ __forceinline__ float mix_log_pdf(const GuideMix& m, float wx, float 
     return mx + __logf(s);
 }
 
+// The same density straight from the outputs, lobe by lobe (no 48-register table).
+template <class Out>
+__device__ __forceinline__ float mix_log_pdf_o(Out o, float ls, float wx, float wy, float wz) {
+    float t[kGLobes], mx = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) {
+        t[k] = lobe_log_term(mix_lobe(o, k, ls), wx, wy, wz);
+        mx = fmaxf(mx, t[k]);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kGLobes; k++) s += __expf(t[k] - mx);
+    return mx + __logf(s);
+}
+
 // Basis around unit n (Duff et al., JCGT 2017).
 __device__ __forceinline__ void onb(float nx, float ny, float nz, float3& t1, float3& t2) {
     const float s = nz >= 0.0f ? 1.0f : -1.0f;
@@ -268,6 +318,20 @@ __device__ __forceinline__ void onb(float nx, float ny, float nz, float3& t1, fl
     const float b = nx * ny * a;
     t1 = make_float3(1.0f + s * nx * nx * a, s * b, -s * nx);
     t2 = make_float3(b, s + ny * ny * a, -ny);
+}
+
+// (u1, u2) invert the vMF of concentration kap about unit mean (mx, my, mz).
+__device__ __forceinline__ float3 vmf_sample(float mx, float my, float mz, float kap, float u1, float u2) {
+    // cos of the angle to the mean: 1 + log((1 - v) + v e^{-2 kappa}) / kappa, v = 1 - u1
+    const float v = 1.0f - u1;
+    const float cz = fminf(fmaxf(1.0f + __logf(u1 + v * __expf(-2.0f * kap)) / kap, -1.0f), 1.0f);
+    const float sz = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
+    float sp, cp;
+    __sincosf(6.283185307179586f * u2, &sp, &cp);
+    float3 t1, t2;
+    onb(mx, my, mz, t1, t2);
+    return make_float3(cz * mx + sz * (cp * t1.x + sp * t2.x), cz * my + sz * (cp * t1.y + sp * t2.y),
+                       cz * mz + sz * (cp * t1.z + sp * t2.z));
 }
 
 // u0 picks the lobe through the CDF of pi; (u1, u2) invert the vMF about its mean.
@@ -289,15 +353,21 @@ __device__ __forceinline__ float3 mix_sample(const GuideMix& m, float u0, float 
         mz += sel * m.mz[j];
         kap += sel * m.kappa[j];
     }
-    const float v = 1.0f - u1;
-    const float cz = fminf(fmaxf(1.0f + log1pf(v * expm1f(-2.0f * kap)) / kap, -1.0f), 1.0f);
-    const float sz = sqrtf(fmaxf(0.0f, 1.0f - cz * cz));
-    float sp, cp;
-    __sincosf(6.283185307179586f * u2, &sp, &cp);
-    float3 t1, t2;
-    onb(mx, my, mz, t1, t2);
-    return make_float3(cz * mx + sz * (cp * t1.x + sp * t2.x), cz * my + sz * (cp * t1.y + sp * t2.y),
-                       cz * mz + sz * (cp * t1.z + sp * t2.z));
+    return vmf_sample(mx, my, mz, kap, u1, u2);
+}
+
+// The same sampler straight from the outputs (an indexed read of the picked lobe).
+template <class Out>
+__device__ __forceinline__ float3 mix_sample_o(Out o, float ls, float u0, float u1, float u2) {
+    int k = kGLobes - 1;
+    float cdf = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kGLobes - 1; j++) {
+        cdf += __expf(o(5 * j) - ls);
+        if (u0 < cdf && k == kGLobes - 1) k = j;
+    }
+    const GuideLobe l = mix_lobe(o, k, ls);
+    return vmf_sample(l.mx, l.my, l.mz, l.kappa, u1, u2);
 }
 
 // ---- device-side guide state shared by the render and training kernels -----------------------------
@@ -326,8 +396,8 @@ struct GuideState {
     uint32_t perm_off;
     float loss_sum;        // sum over the batch of -(F/p) log q
     float weight_sum;
-    float gscale;          // this step's gradient scale (norm clipping)
-    float gnorm_ema;       // running mean of the clipped gradient norms (0 before the first step)
+    float gnorm2;          // squared norm of the last training step's mean gradient (0: none yet)
+    float gnorm_ema;       // running mean of the clipped gradient norms of earlier steps (0: none)
 };
 
 }  // namespace spk

@@ -7,8 +7,10 @@
 //                       hidden activations and the back-propagated deltas (bf16, unit-major)
 //   k_guide_dw          weight gradients dW = delta * a^T as MFMA tiles over sample ranges
 //                       (one partial per range, no atomics)
-//   k_guide_adam        sums the partials, Adam update of the fp32 master weights, and
-//                       rewrites the bf16 forward/backward images.
+//   k_guide_gsum        sums the partials, and per block the squared gradient norm
+//   k_guide_gnorm       the blocks' squared norms in a fixed order (norm clipping)
+//   k_guide_adam        Adam update of the fp32 master weights (the step clipped to a bound on
+//                       its gradient norm), and rewrites the bf16 forward/backward images.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -21,6 +23,13 @@
 #include "spk_sampler.h"
 
 namespace spk {
+
+// Gradient-norm clipping bound, in running means of earlier steps' norms (0 = off, the
+// default); see k_guide_gsum.
+#ifndef SPK_GUIDE_CLIP
+#define SPK_GUIDE_CLIP 0
+#endif
+constexpr float kGuideClip = SPK_GUIDE_CLIP;
 
 // ---- parameter images ------------------------------------------------------------------------------
 __device__ void write_param_images(int p, float th, uint16_t* wf, uint16_t* wb, float* bias) {
@@ -146,6 +155,12 @@ __global__ void k_guide_step_begin(GuideState* gs, uint32_t batch, uint32_t rec_
     gs->perm_off = n > 0 ? h0 % n : 0u;
     gs->loss_sum = 0.0f;
     gs->weight_sum = 0.0f;
+    // the last training step's gradient norm joins the running mean (clipped as it was)
+    if (gs->gnorm2 > 0.0f) {
+        const float norm = sqrtf(gs->gnorm2), ema = gs->gnorm_ema;
+        gs->gnorm_ema = ema > 0.0f ? 0.9f * ema + 0.1f * fminf(norm, kGuideClip * ema) : norm;
+        gs->gnorm2 = 0.0f;
+    }
 }
 
 __device__ __forceinline__ float safe_ratio(float a, float b) { return b > 0.0f ? a / b : 0.0f; }
@@ -457,40 +472,44 @@ __global__ __launch_bounds__(64) void k_guide_dw(TrainBufs tb, const GuideState*
     }
 }
 
-// Gradient-norm clipping of a training step: one block sums the squared mean gradient over
-// all parameters; a step whose norm exceeds kGuideClip times the running mean of earlier
-// (clipped) norms is scaled back to that bound.  A record whose radiance estimate F/p is far
-// above the rest can otherwise steer a whole Adam step.  The first step is never clipped.
-#ifndef SPK_GUIDE_CLIP
-#define SPK_GUIDE_CLIP 0
-#endif
-constexpr float kGuideClip = SPK_GUIDE_CLIP;
-constexpr int kGnormBlock = 1024;
+// Gradient-norm clipping of a training step.  k_guide_gsum folds the per-range partial
+// gradients into the first range's slot (same summation order as an unclipped step) and
+// accumulates the squared norm of the mean gradient; Adam scales a step whose norm exceeds
+// kGuideClip times the running mean of earlier (clipped) norms back to that bound, and the
+// next step's set-up folds the norm into the running mean.  One record whose radiance
+// estimate F/p is far above the rest can steer a whole Adam step: with a bound of 3, ten
+// training runs on C3 at 640x360 gave variance gains of 1.9-3.3 where two of ten runs without
+// it stopped near 1.3.  But the rare high-F/p records are also the caustic paths the guide is
+// there to learn: the same bound lengthened C3's time to relMSE 1e-3 (guided/plain 0.67 ->
+// 0.73 over four seeds) and left C4's sun caustics unlearnt (view 0 relMSE at 256 spp 0.06 ->
+// 0.52), so it is off.  The first step is never clipped.
 
-__global__ __launch_bounds__(kGnormBlock) void k_guide_gnorm(const float* __restrict__ partial, int splits,
-                                                             GuideState* gs) {
-    __shared__ float red[kGnormBlock / 64];
+__global__ __launch_bounds__(256) void k_guide_gsum(float* partial, int splits, const GuideState* gs,
+                                                    float* gnorm_part) {
+    __shared__ float red[4];
     if (!gs->active) return;
-    const float inv_b = 1.0f / (float)gs->b_eff;
-    float acc = 0.0f;
-    for (int p = threadIdx.x; p < kGParams; p += kGnormBlock) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float sq = 0.0f;
+    if (p < kGParams) {
         float s = 0.0f;
         for (int k = 0; k < splits; k++) s += partial[(size_t)k * kGParams + p];
-        s *= inv_b;
-        acc += s * s;
+        partial[p] = s;
+        const float g = s / (float)gs->b_eff;
+        sq = g * g;
     }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.0f;
-        for (int i = 0; i < kGnormBlock / 64; i++) t += red[i];
-        const float norm = sqrtf(t), ema = gs->gnorm_ema;
-        const float bound = ema > 0.0f ? kGuideClip * ema : 3.0e38f;
-        const float kept = fminf(norm, bound);
-        gs->gscale = norm > bound ? bound / norm : 1.0f;
-        gs->gnorm_ema = ema > 0.0f ? 0.9f * ema + 0.1f * kept : norm;
-    }
+    if (threadIdx.x == 0) gnorm_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// The blocks' partials summed in a fixed order (no float atomics: training stays
+// bit-reproducible).
+__global__ void k_guide_gnorm(const float* __restrict__ gnorm_part, GuideState* gs) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !gs->active) return;
+    float t = 0.0f;
+    for (int i = 0; i < kGnormBlocks; i++) t += gnorm_part[i];
+    gs->gnorm2 = t;
 }
 
 __global__ void k_guide_adam(float* params, float* m, float* v, const float* __restrict__ partial, int splits,
@@ -505,7 +524,10 @@ __global__ void k_guide_adam(float* params, float* m, float* v, const float* __r
         grad_out[p] = gr;
         return;
     }
-    if (kGuideClip > 0.0f) gr *= gs->gscale;
+    if (kGuideClip > 0.0f) {
+        const float norm = sqrtf(gs->gnorm2), bound = kGuideClip * gs->gnorm_ema;
+        if (bound > 0.0f && norm > bound) gr *= bound / norm;
+    }
     const float t = (float)gs->t;
     const float mm = cfg.beta1 * m[p] + (1.0f - cfg.beta1) * gr;
     const float vv = cfg.beta2 * v[p] + (1.0f - cfg.beta2) * gr * gr;
@@ -606,11 +628,15 @@ int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNe
     const uint32_t cps = kGuideColsPerSplit;
     const uint32_t splits = (tb.ncols + cps - 1) / cps;
     hipLaunchKernelGGL(k_guide_dw, dim3(8, splits), dim3(64), 0, st, tb, (const GuideState*)gb.gs, gb.partial, cps);
-    if (kGuideClip > 0.0f && !grad_out)
-        hipLaunchKernelGGL(k_guide_gnorm, dim3(1), dim3(kGnormBlock), 0, st, (const float*)gb.partial, (int)splits, gb.gs);
+    const bool clip = kGuideClip > 0.0f && !grad_out;
+    if (clip) {
+        hipLaunchKernelGGL(k_guide_gsum, dim3(kGnormBlocks), dim3(256), 0, st, gb.partial, (int)splits,
+                           (const GuideState*)gb.gs, gb.gnorm_part);
+        hipLaunchKernelGGL(k_guide_gnorm, dim3(1), dim3(64), 0, st, (const float*)gb.gnorm_part, gb.gs);
+    }
     hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gb.params, gb.m, gb.v,
-                       (const float*)gb.partial, (int)splits, (const GuideState*)gb.gs, cfg, grad_out, gb.wf, gb.wb,
-                       gb.bias);
+                       (const float*)gb.partial, clip ? 1 : (int)splits, (const GuideState*)gb.gs, cfg, grad_out, gb.wf,
+                       gb.wb, gb.bias);
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
 }

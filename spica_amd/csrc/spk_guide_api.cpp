@@ -24,6 +24,7 @@ spk::GuideBufsDev spk_guide::bufs() const {
     b.wb = wb.as<uint16_t>();
     b.bias = bias.as<float>();
     b.partial = partial.as<float>();
+    b.gnorm_part = gnorm_part.as<float>();
     b.gs = state.as<GuideState>();
     b.grid = cus;
     return b;
@@ -192,7 +193,7 @@ int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out) {
     if ((rc = g->params.alloc(kGParams * 4)) || (rc = g->m.alloc(kGParams * 4)) || (rc = g->v.alloc(kGParams * 4)) ||
         (rc = g->wf.alloc(kGFwdSize * 2)) || (rc = g->wb.alloc(kGBwdSize * 2)) || (rc = g->bias.alloc(kGBiasSize * 4)) ||
         (rc = g->partial.alloc(splits * kGParams * 4)) || (rc = g->state.alloc(sizeof(GuideState))) ||
-        (rc = g->grad.alloc(kGParams * 4)))
+        (rc = g->grad.alloc(kGParams * 4)) || (rc = g->gnorm_part.alloc(kGnormBlocks * 4)))
         return rc;
     for (int k = 0; k < 3; k++)
         if ((rc = g->tb[k].alloc(B * 16))) return rc;
@@ -218,7 +219,7 @@ int spk_guide_destroy(spk_guide* g) {
         (void)hipStreamDestroy(g->stream);
     }
     DevBuf* b[] = {&g->params, &g->m, &g->v, &g->wf, &g->wb, &g->bias, &g->partial, &g->state, &g->grad,
-                   &g->tb[0], &g->tb[1], &g->tb[2], &g->E, &g->act, &g->dz};
+                   &g->gnorm_part, &g->tb[0], &g->tb[1], &g->tb[2], &g->E, &g->act, &g->dz};
     for (DevBuf* x : b) x->release();
     delete g;
     return SPK_OK;

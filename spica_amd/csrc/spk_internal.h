@@ -223,10 +223,12 @@ struct GuideBufsDev {
     uint16_t* wb;
     float* bias;
     float* partial;     // [splits][kGParams]
+    float* gnorm_part;  // [kGnormBlocks] squared-norm partials of one step's gradient
     GuideState* gs;
     int grid;           // persistent blocks for the training kernel (compute units)
 };
 constexpr uint32_t kGuideColsPerSplit = 1024;
+constexpr int kGnormBlocks = (kGParams + 255) / 256;
 
 int launch_guide_images(const float* params, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t st);
 int launch_guide_eval(int mode, const uint16_t* wf, const float* bias, int64_t n, const float4* x, const float4* aux,
@@ -248,7 +250,7 @@ struct spk_guide {
     int device = 0;
     hipStream_t stream = nullptr;       // for the stand-alone entry points
     spk_guide_opts opts;
-    spk::DevBuf params, m, v, wf, wb, bias, partial, state, grad;
+    spk::DevBuf params, m, v, wf, wb, bias, partial, state, grad, gnorm_part;
     spk::DevBuf tb[3], E, act, dz;
     uint32_t bcap = 0;                  // batch capacity (samples)
     int cus = 256;                      // compute units of the device

@@ -1497,6 +1497,25 @@ __device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
     return (b.lobe == LOBE_LAMBERT && !black(b.R)) || (b.lobe == LOBE_MICROREFL && b.dist == 0);
 }
 
+// The direction bsdf_sample draws at a guided vertex (Lambert or GGX reflection) from the same
+// uniforms, without its f and pdf: the guide kernel evaluates both once for whichever strategy
+// drew the direction.  False where bsdf_sample finds no direction.
+__device__ bool guide_bsdf_dir(const Bsdf& b, float3 woW, float u0, float u1, float3& wiW) {
+    const float3 wo = normalize(to_local(b, woW));
+    if (wo.z == 0.0f) return false;
+    float3 wi;
+    if (b.lobe == LOBE_LAMBERT) {
+        wi = cosine_hemisphere(u0, u1);
+        if (wo.z < 0.0f) wi.z *= -1.0f;
+    } else {
+        const float3 wh = mf_sample_wh<false>(b, wo, u0, u1);
+        wi = sub(mul(wh, 2.0f * dot(wh, wo)), wo);
+        if (!(wo.z * wi.z > 0.0f)) return false;
+    }
+    wiW = normalize(to_world(b, wi));
+    return true;
+}
+
 // ---- shade ---------------------------------------------------------------------------------------
 // Shading runs in blocks of its own size, independent of the traversal kernels' block:
 // the queue appends are aggregated per block (block_append).
@@ -1827,8 +1846,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BATCH ? 7
 // mixture frac * guide + (1 - frac) * BSDF (path.cc:83-94 with the proposal added), applies
 // Russian roulette (path.cc:116-121) and, in training passes, records (inputs, w_i,
 // radiance so far, throughput) for the network.
+// One 512-thread block per CU: the weights (58 KB) and, per wave, the network outputs staged in
+// fp32 (64 x 41 floats) take 144 KB of LDS.  The mixture reads its parameters lobe by lobe from
+// the staged outputs.  (Staging in fp16 with 768-thread blocks, 3 waves per SIMD at 168 VGPRs,
+// measured the same: 7.40 against 7.32 G vertices/s after training on C3; the kernel is bound
+// by VALU issue, not latency.)
 constexpr int kGuideBlock = 512;
-constexpr int kGuideObStride = 41;
+constexpr int kGuideObStride = 41;  // floats per lane: 40 outputs, padded to an odd count
 
 __device__ __forceinline__ uint32_t guide_dim(int depth) { return 4u + 8u * (uint32_t)(kMaxDepthCap + depth) + 1u; }
 
@@ -1850,7 +1874,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
                                                       GuideNetDev net, int slot, int oslot) {
     __shared__ __attribute__((aligned(16))) uint16_t w_lds[kGFwdSize];
     __shared__ float b_lds[kGBiasSize];
-    __shared__ float obuf[kGuideBlock / 64][64 * kGuideObStride];
+    __shared__ __attribute__((aligned(16))) float obuf[kGuideBlock / 64][64 * kGuideObStride];
     PassState* ps = rb.ps;
     const uint32_t count = ps->countG[slot];
     if (count == 0) return;
@@ -1864,7 +1888,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
     const uint32_t sample_base = ps->sample_base;
     const int nxt = oslot & 1;
     const int lane = __lane_id(), wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    float* ob = obuf[wv];
+    float* ob = obuf[wv];  // the inputs (8 floats per lane), then the network's outputs
     __shared__ uint32_t app_lds[2 * 2 * (kGuideBlock / 64 + 1)];
     uint32_t parity = 0;
     // block-uniform trip count: the appends below synchronise the block
@@ -1899,7 +1923,8 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int u = 16 * mt + 4 * g + k;
-                    if (u < kGOut) ob[(16 * ct + r) * kGuideObStride + u] = out[mt][ct][k];
+                    if (u < kGOut)
+                        ob[(16 * ct + r) * kGuideObStride + u] = out[mt][ct][k];
                 }
         wave_lds_sync();
         bool pushA = false, pushR = false;
@@ -1907,9 +1932,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
         uint32_t nA_flags = 0, path = 0, pfield = 0, rsample = 0;
         int depth = 0;
         if (active) {
-            const float* o = ob + lane * kGuideObStride;
-            GuideMix mix;
-            mix_from_outputs([&](int u) { return o[u]; }, mix);
+            const float* orow = ob + lane * kGuideObStride;
+            const auto o = [orow](int u) { return orow[u]; };
+            const float ls = mix_log_norm(o);
             path = __float_as_uint(G0.w);
             pfield = __float_as_uint(G2.w);
             const uint32_t pixel = BATCH ? batch_pixel(pfield) : pfield;
@@ -1929,18 +1954,12 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide(SceneDev sc, const Config
             const float us = sample_1d(kind, seed, pixel, sample, guide_dim(depth));
             float3 wi = mk(0, 0, 1);
             bool ok = true;
-            if (us < net.frac) {
-                wi = mix_sample(mix, us / net.frac, uc.x, uc.y);
-            } else {
-                float pdf_b = 0.0f;
-                int st = 0;
-                float3 fb = bsdf_sample<false>(b, wo, wi, uc.x, uc.y, pdf_b, BX_ALL, st, 0.5f);
-                ok = pdf_b > 0.0f && !black(fb);
-            }
+            if (us < net.frac) wi = mix_sample_o(o, ls, us / net.frac, uc.x, uc.y);
+            else ok = guide_bsdf_dir(b, wo, uc.x, uc.y, wi);
             if (ok) {
                 const float3 f = bsdf_f<false>(b, wo, wi, BX_ALL);
                 const float pb = bsdf_pdf<false>(b, wo, wi, BX_ALL);
-                const float pdf = (1.0f - net.frac) * pb + net.frac * __expf(mix_log_pdf(mix, wi.x, wi.y, wi.z));
+                const float pdf = (1.0f - net.frac) * pb + net.frac * __expf(mix_log_pdf_o(o, ls, wi.x, wi.y, wi.z));
                 if (!black(f) && pdf > 0.0f) {
                     const float cw = fabsf(dot(wi, b.n));
                     float3 nb = cmul(beta, mul(f, cw / pdf));
