@@ -117,10 +117,12 @@ def test_lbvh_trace_parity_meshes(gpu, oracle, mesh):
     _trace_parity(gpu, oracle, s, org, dirv, 1e-3, builder="lbvh")
 
 
-def test_lbvh_trace_parity_c1(gpu, oracle):
+@pytest.mark.parametrize("treelet_passes", [0, 6])
+def test_lbvh_trace_parity_c1(gpu, oracle, treelet_passes):
+    """The Morton tree as built and after treelet restructuring (the default) answer alike."""
     w = configs.c1_spheres()
     org, dirv = ray_set(w.scene, w.camera, oracle.scene(w.scene), 2_000_000, seed=6)
-    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-4, builder="lbvh")
+    _trace_parity(gpu, oracle, w.scene, org, dirv, 1e-4, builder="lbvh", treelet_passes=treelet_passes)
 
 
 def test_lbvh_degenerate_scenes(gpu, oracle):
