@@ -96,7 +96,7 @@ int spk_trace_noalloc(spk_scene* scene, const float* org, const float* dir, cons
 
 /* Tuning knobs that spica does not have (its CPU loop needs none). */
 typedef struct spk_render_opts {
-    int64_t queue_size;     /* paths in flight per pass (0 = auto: 2^21) */
+    int64_t queue_size;     /* paths in flight per pass, 64..2^27 (0 = auto: 2^21) */
     int32_t block_size;     /* 64, 128 or 256 (0 = auto) */
     int32_t use_graph;      /* capture each pass into a hipGraph and replay it */
     int32_t profile;        /* HIP events around every closest-hit launch (1), and also around

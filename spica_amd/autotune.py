@@ -4,15 +4,15 @@ Passes regenerate paths: a pass covers up to 2^28 paths (whole samples of the fi
 `queue` of them in flight, starting the pass's next camera path in every slot a finished path
 frees, so every launch works on a full queue of mixed-depth paths.  More paths in flight hide
 more latency per launch and need fewer launches, at ~272 B of queue memory each; the
-candidates span 2^20..2^26 for single renders and 2^20..2^23 for batches.  The tuner times a
-pass per candidate, keeps the fastest by traced rays per second and caches the choice per
-(scene size, film, depth, device).
+candidates span 2^20..2^27 (the largest queue a render takes) for single renders and
+2^20..2^23 for batches.  The tuner times a pass per candidate, keeps the fastest by traced rays
+per second and caches the choice per (scene size, film, depth, device).
 """
 import json
 import os
 import time
 
-DEFAULT_QUEUES = tuple(1 << k for k in range(20, 27))
+DEFAULT_QUEUES = tuple(1 << k for k in range(20, 28))
 BATCH_QUEUES = tuple(1 << k for k in range(20, 24))
 PASS_PATHS = 1 << 28  # paths per regenerating pass
 DEFAULT_BLOCKS = (64, 128, 256)

@@ -803,7 +803,7 @@ int spk_render_create(spk_scene* s, const spk_camera_desc* cam, const spk_render
     if (opts) o = *opts;
     if (o.queue_size <= 0) o.queue_size = int64_t(1) << 21;
     if (o.queue_size < 64) o.queue_size = 64;
-    if (o.queue_size > (int64_t(1) << 26)) o.queue_size = int64_t(1) << 26;
+    if (o.queue_size > (int64_t(1) << 27)) o.queue_size = int64_t(1) << 27;
     if (o.block_size != 0 && o.block_size != 64 && o.block_size != 128 && o.block_size != 256) {
         set_error("spk_render_create: block_size %d not one of 64, 128, 256", o.block_size);
         return SPK_ERR_INVALID;
