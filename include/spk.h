@@ -177,8 +177,9 @@ int spk_render_stats_get(spk_render* render, spk_render_stats* out);
  * direction whose 40 outputs define an 8-lobe von Mises-Fisher mixture over incident
  * directions.  Attached to a render it is trained online by Adam on the render's own
  * radiance samples during the first `train_spp` samples per pixel, and continuations at
- * diffuse and glossy vertices are drawn from guide_fraction * guide + (1 - guide_fraction)
- * * BSDF with the mixture density in the weight, so the estimator stays unbiased.
+ * diffuse vertices (and at microfacet reflections of alpha >= glossy_min_alpha) of the first
+ * `guided_depth` bounces are drawn from guide_fraction * guide + (1 - guide_fraction) * BSDF
+ * with the mixture density in the weight, so the estimator stays unbiased.
  *
  * Network inputs x (7 floats per sample, in [0,1]): position scaled to the scene box (3),
  * then for the normal and the outgoing direction ((v.z + 1)/2, atan2(v.y, v.x)/2pi + 1/2). */
@@ -190,10 +191,17 @@ typedef struct spk_guide_opts {
     int32_t steps_per_pass;  /* Adam steps after each training round (default 4) */
     int32_t train_spp;       /* samples per pixel of online training (default 64) */
     float learning_rate;     /* default 1e-2 */
-    float beta1, beta2, epsilon;  /* default 0.9, 0.999, 1e-8 */
+    float beta1, beta2, epsilon;  /* default 0.9, 0.95, 1e-8 */
     float guide_fraction;    /* probability of sampling the proposal at a vertex (default 0.5) */
     int32_t train_pass_spp;  /* samples per pixel per training round (default 4); each round
                               * keeps 1/train_pass_spp of its vertices as training records */
+    float glossy_min_alpha;  /* microfacet reflections are guided only where their roughness
+                              * alpha is at least this (default 0.5; 0 guides all of them):
+                              * narrower lobes are left to BSDF sampling */
+    int32_t guided_depth;    /* vertices of bounces 0 .. guided_depth - 1 are guided (default 3,
+                              * >= 1): a BSDF-drawn direction the guide holds dark weighs
+                              * 1/(1 - guide_fraction) times its plain weight, and over many
+                              * guided bounces that factor compounds */
 } spk_guide_opts;
 
 int spk_guide_opts_default(spk_guide_opts* out);

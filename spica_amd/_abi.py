@@ -200,6 +200,8 @@ class SpkGuideOpts(ctypes.Structure):
         ("epsilon", ctypes.c_float),
         ("guide_fraction", ctypes.c_float),
         ("train_pass_spp", ctypes.c_int32),
+        ("glossy_min_alpha", ctypes.c_float),
+        ("guided_depth", ctypes.c_int32),
     ]
 
 

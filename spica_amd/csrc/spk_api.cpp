@@ -1220,13 +1220,18 @@ int spk_render_set_guide(spk_render* r, spk_guide* g) {
     if (g) SPK_CHECK_HIP(hipStreamSynchronize(g->stream));
     r->guide = g;
     r->guide_trained = 0;
+    // read by the shading kernels of guided runs (batches launch without re-uploading the config)
+    r->cfg.guide_min_alpha = g ? g->opts.glossy_min_alpha : 0.0f;
+    r->cfg.guide_depth = g ? g->opts.guided_depth : 0;
+    SPK_CHECK_HIP(hipMemcpy(r->cfg_dev.p, &r->cfg, sizeof(ConfigDev), hipMemcpyHostToDevice));
     if (!g || r->qg[0].p) return SPK_OK;
     int rc;
     const size_t Q = r->inflight;
     for (int k = 0; k < 4; k++)
         if ((rc = r->qg[k].alloc(Q * 16))) return rc;
-    // training records of one round: about one kept vertex per pixel, room for four
-    r->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)r->cfg.npix * 4u, uint64_t(1) << 24);
+    // training records of one round: about one kept vertex per pixel, room for four, up to 2^25
+    // (160 B each with the sorted copy: 5.4 GB at the cap, which an 8-view 1080p batch reaches)
+    r->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)r->cfg.npix * 4u, uint64_t(1) << 25);
     for (int k = 0; k < 5; k++)
         if ((rc = r->rec[k].alloc((size_t)r->rec_cap * 16)) || (rc = r->rec_s[k].alloc((size_t)r->rec_cap * 16))) return rc;
     size_t sort_bytes = 0;

@@ -130,6 +130,8 @@ struct ConfigDev {
     int wide;               // some scene needs the wide-material shading kernel
     int regen;              // spent paths are replaced by the pass's next camera paths
     uint32_t inflight;      // regen: queue capacity (paths in flight); the pass holds up to `queue`
+    float guide_min_alpha;  // guided renders: microfacet reflections below this alpha are not guided
+    int guide_depth;        // guided renders: vertices of bounces 0 .. guide_depth - 1 are guided
 };
 
 // Device-side per-pass state: queue lengths and work counters per bounce.

@@ -149,10 +149,17 @@ int spk_guide_opts_default(spk_guide_opts* o) {
     o->train_spp = 64;
     o->learning_rate = 1.0e-2f;
     o->beta1 = 0.9f;
-    o->beta2 = 0.999f;
+    // A short second-moment memory: a record whose radiance estimate F/p is far above the rest
+    // (a caustic path found by BSDF sampling) gives one step a gradient orders of magnitude
+    // larger than the others.  With beta2 = 0.999 that step moves every weight by ~3 lr and
+    // then holds the second moment up for hundreds of steps, so the steps after it barely move;
+    // with 0.95 the spike's own step is ~0.45 lr and training recovers within ~20 steps.
+    o->beta2 = 0.95f;
     o->epsilon = 1.0e-8f;
     o->guide_fraction = 0.5f;
     o->train_pass_spp = 4;
+    o->glossy_min_alpha = 0.5f;
+    o->guided_depth = 3;
     return SPK_OK;
 }
 
@@ -176,7 +183,8 @@ int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out) {
         return SPK_ERR_INVALID;
     }
     if (o.steps_per_pass < 0 || o.train_spp < 0 || o.train_pass_spp < 1 || !(o.learning_rate > 0.0f) || !(o.guide_fraction >= 0.0f) ||
-        !(o.guide_fraction < 1.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) || !(o.beta2 >= 0.0f && o.beta2 < 1.0f)) {
+        !(o.guide_fraction < 1.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) || !(o.beta2 >= 0.0f && o.beta2 < 1.0f) ||
+        !(o.glossy_min_alpha >= 0.0f && o.glossy_min_alpha < 1.0e30f) || o.guided_depth < 1) {
         set_error("spk_guide_create: bad options");
         return SPK_ERR_INVALID;
     }

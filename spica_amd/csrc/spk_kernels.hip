@@ -1494,8 +1494,12 @@ __device__ __forceinline__ uint32_t hidden_dim_path(int depth, int which) {
 // Vertices whose continuation the learned proposal samples: Lambert and GGX reflection
 // (the guide kernel is built for the four basic lobes; the plastics' sample() does not
 // even agree with their f() and pdf(), roughplastic.cc:49-97).
-__device__ __forceinline__ bool guide_eligible(const Bsdf& b) {
-    return (b.lobe == LOBE_LAMBERT && !black(b.R)) || (b.lobe == LOBE_MICROREFL && b.dist == 0);
+// Vertices whose continuation the proposal may draw: diffuse ones, and microfacet reflections
+// (GGX) at least `min_alpha` rough.  A narrow glossy lobe turns with the outgoing direction
+// faster than the network's inputs resolve, so the mixture wastes its share of the samples
+// there and BSDF sampling alone does better.
+__device__ __forceinline__ bool guide_eligible(const Bsdf& b, float min_alpha) {
+    return (b.lobe == LOBE_LAMBERT && !black(b.R)) || (b.lobe == LOBE_MICROREFL && b.dist == 0 && b.alpha >= min_alpha);
 }
 
 // The direction bsdf_sample draws at a guided vertex (Lambert or GGX reflection) from the same
@@ -1538,6 +1542,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(shade_wav
     // directlighting.cc:21-57: emission at the first hit only, NEE at every hit, and the
     // recursion only through specular reflection (see the continuation below)
     const bool direct = cfgp->integrator == SPK_INTEGRATOR_DIRECT;
+    const float guide_min_alpha = GUIDED ? cfgp->guide_min_alpha : 0.0f;
+    const int guide_depth = GUIDED ? cfgp->guide_depth : 0;
     const uint32_t sample_base = ps->sample_base;
     const int cur = slot & 1, nxt = oslot & 1;
     __shared__ uint32_t app_lds[2 * 4 * (BLOCK / 64 + 1)];
@@ -1688,7 +1694,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(shade_wav
                                 nA_flags = (flags & ~0x7fu) | (uint32_t)(depth + 1) | (1u << 6);
                             }
                         }
-                    } else if (GUIDED && guide_eligible(b)) {
+                    } else if (GUIDED && depth < guide_depth && guide_eligible(b, guide_min_alpha)) {
                         pushG = true;
                         gq0 = make_float4(p.x, p.y, p.z, __uint_as_float(path));
                         gq1 = make_float4(wo.x, wo.y, wo.z, __uint_as_float(flags));

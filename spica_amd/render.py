@@ -242,7 +242,10 @@ class Batch:
 def guide_opts(**kw):
     o = _abi.SpkGuideOpts()
     _abi.check(_abi.load().spk_guide_opts_default(ctypes.byref(o)), "spk_guide_opts_default")
+    names = {f[0] for f in o._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown guide option {k!r}")
         setattr(o, k, v)
     return o
 

@@ -75,3 +75,57 @@ def test_scene_build_options_are_checked():
         rc = lib.spk_scene_create_ex(ctypes.byref(d), ctypes.byref(bo), ctypes.byref(h))
         assert rc == _abi.SPK_ERR_INVALID and b"build options" in lib.spk_last_error()
     assert lib.spk_scene_build_opts_default(None) == _abi.SPK_ERR_INVALID
+
+
+def _c_name(cls):
+    return re.sub(r"(?<!^)([A-Z])", r"_\1", cls.__name__).lower()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror in the bindings has the size and field offsets of its C struct in
+    include/ (compiled with the host compiler), so no option or statistic is read from the
+    wrong bytes."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    structs = [v for v in vars(_abi).values()
+               if isinstance(v, type) and issubclass(v, ctypes.Structure) and v.__name__.startswith("Spk")]
+    assert len(structs) >= 13
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "spk.h"', "int main(void) {"]
+    for cls in structs:
+        c = _c_name(cls)
+        lines.append(f'    printf("{c} %zu\\n", sizeof({c}));')
+        for f in cls._fields_:
+            lines.append(f'    printf("{c}.{f[0]} %zu\\n", offsetof({c}, {f[0]}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                       text=True).stdout.splitlines())
+    for cls in structs:
+        c = _c_name(cls)
+        assert int(got[c]) == ctypes.sizeof(cls), c
+        for f in cls._fields_:
+            assert int(got[f"{c}.{f[0]}"]) == getattr(cls, f[0]).offset, f"{c}.{f[0]}"
+
+
+def test_guide_option_defaults():
+    from spica_amd import render
+
+    o = render.guide_opts()
+    assert (o.batch_size, o.train_spp, o.train_pass_spp) == (1 << 18, 64, 4)
+    assert abs(o.learning_rate - 1e-2) < 1e-9 and o.guide_fraction == 0.5 and o.glossy_min_alpha == 0.5
+    assert abs(o.beta2 - 0.95) < 1e-7 and o.guided_depth == 3
+    try:
+        render.guide_opts(glossy_min_alpa=0.0)
+    except TypeError as e:
+        assert "glossy_min_alpa" in str(e)
+    else:
+        raise AssertionError("a misspelt guide option was accepted")

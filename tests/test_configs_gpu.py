@@ -93,12 +93,15 @@ def _check(name, gpu_film, cpu_film, mean, guided, sem=None):
 
 
 def _render(gs, wl, spp, guided, queue=1 << 24):
+    """guided: False, True (the default guide), or "all-glossy" (every GGX reflection guided,
+    however narrow its lobe, at every bounce)."""
     from spica_amd import render
 
     gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=spp), queue_size=queue, block_size=64)
     g = None
     if guided:
-        g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8)
+        extra = dict(glossy_min_alpha=0.0, guided_depth=64) if guided == "all-glossy" else {}
+        g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8, **extra)
         gr.set_guide(g)
     gr.run(spp, sync=True)
     film = gr.film()
@@ -133,14 +136,14 @@ def _dump(name, **arrays):
         np.savez_compressed(os.path.join(out, name + ".npz"), **arrays)
 
 
-@pytest.mark.parametrize("guided", [False, True])
+@pytest.mark.parametrize("guided", [False, True, "all-glossy"])
 def test_c3_full_size(c3, guided):
     wl, gs, d, cpu = c3
     film = _render(gs, wl, 1024, guided)
-    _dump(f"c3_{int(guided)}", gpu=np.stack(_at(film, d["pixels"])[:2]), gcount=_at(film, d["pixels"])[2],
+    _dump(f"c3_{guided}", gpu=np.stack(_at(film, d["pixels"])[:2]), gcount=_at(film, d["pixels"])[2],
           cpu=np.stack(cpu[:2]), ccount=cpu[2], mean=d["mean"], sem=d["sem"])
-    _check(f"C3 {'guided' if guided else 'plain'} 1024 spp", _at(film, d["pixels"]), cpu, d["mean"], guided,
-           d["sem"])
+    label = {False: "plain", True: "guided", "all-glossy": "guided (every GGX vertex, every bounce)"}[guided]
+    _check(f"C3 {label} 1024 spp", _at(film, d["pixels"]), cpu, d["mean"], bool(guided), d["sem"])
 
 
 @pytest.fixture(scope="module")

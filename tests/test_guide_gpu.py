@@ -98,17 +98,33 @@ def test_gradient_matches_oracle(guide):
             assert blk < 3e-2, (lo, hi, blk)
 
 
-def test_adam_step_matches_oracle(guide):
+def test_adam_steps_match_oracle(guide):
+    """Eight Adam steps from a fresh state, each on another batch, against the oracle's
+    recursion with the guide's own hyper-parameters.  A first step alone is lr * sign(g)
+    whatever beta1 and beta2 are; over eight steps on differing gradients the moments' decay
+    shows: the same steps with beta2 = 0.999 instead of the default 0.95 end far outside the
+    tolerance."""
+    o = guide.opts
+    assert abs(o.beta2 - 0.95) < 1e-7
     th = _params(14)
-    guide.set_params(th)
-    x, d, w = _batch(2048, 4)
-    _, g_gpu = guide.grad(x, d, w)
-    guide.train(x, d, w, steps=1)
-    th1 = guide.params().astype(np.float64)
-    ref, _, _ = G.adam(th, np.zeros_like(th), np.zeros_like(th), 1, g_gpu.astype(np.float64),
-                       lr=1e-2, b1=0.9, b2=0.999, eps=1e-8)
-    assert np.abs(th1 - ref).max() < 1e-6 + 1e-5 * np.abs(ref).max()
-    assert guide.stats()["steps"] == 1
+    guide.set_params(th)  # resets the moments and the step count
+    cur = th.astype(np.float64)
+    m, v = np.zeros_like(cur), np.zeros_like(cur)
+    m_alt, v_alt, alt = m.copy(), v.copy(), cur.copy()
+    hp = dict(lr=o.learning_rate, b1=o.beta1, eps=o.epsilon)
+    for t in range(1, 9):
+        x, d, w = _batch(2048, 40 + t)
+        _, g_gpu = guide.grad(x, d, w)
+        g64 = g_gpu.astype(np.float64)
+        guide.train(x, d, w, steps=1)
+        th1 = guide.params().astype(np.float64)
+        ref, m, v = G.adam(cur, m, v, t, g64, b2=o.beta2, **hp)
+        alt, m_alt, v_alt = G.adam(alt, m_alt, v_alt, t, g64, b2=0.999, **hp)
+        tol = 1e-6 + 1e-5 * np.abs(ref).max()
+        assert np.abs(th1 - ref).max() < tol, t
+        cur = th1
+    assert np.abs(alt - cur).max() > 10 * tol
+    assert guide.stats()["steps"] == 8
 
 
 def test_training_reduces_loss(guide):
