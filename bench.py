@@ -259,7 +259,7 @@ def c3_leg(args, c3_ref):
 
     w3 = configs.c3_occluded_glossy()
     gs3 = render.GpuScene(w3.scene)
-    guide_cfg = dict(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8, learning_rate=1e-2,
+    guide_cfg = dict(batch_size=1 << 18, train_spp=64, train_pass_spp=8, steps_per_pass=8, learning_rate=1e-2,
                      guide_fraction=0.5)
     warm = render.GpuRender(gs3, w3.camera, w3.render_desc(spp=16), queue_size=args.queue, block_size=args.block)
     warm.run(16)
@@ -595,7 +595,7 @@ def main():
                                mrays_per_s=round(lrays / ldt / 1e6, 3)))
         c4_ref = load_reference("c4")
         # the 8 views as one batch, plain and with one guide for all views (trained on the first
-        # 64 samples of every view, rounds of 4 across the views), to heavy_spp samples each
+        # 64 samples of every view, rounds of 8 across the views), to heavy_spp samples each
         res = {}
         for name, guided in (("plain", False), ("guided", True)):
             items = [(gs4, v.camera, args.heavy_spp, v.seed) for v in views]
@@ -603,7 +603,7 @@ def main():
                               block_size=args.block, graph=1)
             g = None
             if guided:
-                g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8)
+                g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=8, steps_per_pass=8)
                 bh.set_guide(g)
             st0 = bh.stats()
             torch.cuda.synchronize()

@@ -188,13 +188,14 @@ typedef struct spk_guide spk_guide;
 typedef struct spk_guide_opts {
     uint32_t seed;           /* weight initialisation */
     int32_t batch_size;      /* training samples per Adam step (default 2^18, multiple of 64) */
-    int32_t steps_per_pass;  /* Adam steps after each training round (default 4) */
+    int32_t steps_per_pass;  /* Adam steps after each training round (default 8) */
     int32_t train_spp;       /* samples per pixel of online training (default 64) */
     float learning_rate;     /* default 1e-2 */
     float beta1, beta2, epsilon;  /* default 0.9, 0.95, 1e-8 */
     float guide_fraction;    /* probability of sampling the proposal at a vertex (default 0.5) */
-    int32_t train_pass_spp;  /* samples per pixel per training round (default 4); each round
-                              * keeps 1/train_pass_spp of its vertices as training records */
+    int32_t train_pass_spp;  /* samples per pixel per training round (default 8: with 8 steps,
+                              * 64 Adam steps over the 64 training spp); each round keeps
+                              * 1/train_pass_spp of its vertices as training records */
     float glossy_min_alpha;  /* microfacet reflections are guided only where their roughness
                               * alpha is at least this (default 0.5; 0 guides all of them):
                               * narrower lobes are left to BSDF sampling */

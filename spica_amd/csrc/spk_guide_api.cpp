@@ -145,7 +145,7 @@ int spk_guide_opts_default(spk_guide_opts* o) {
     memset(o, 0, sizeof(*o));
     o->seed = 1;
     o->batch_size = 1 << 18;
-    o->steps_per_pass = 4;
+    o->steps_per_pass = 8;
     o->train_spp = 64;
     o->learning_rate = 1.0e-2f;
     o->beta1 = 0.9f;
@@ -157,7 +157,7 @@ int spk_guide_opts_default(spk_guide_opts* o) {
     o->beta2 = 0.95f;
     o->epsilon = 1.0e-8f;
     o->guide_fraction = 0.5f;
-    o->train_pass_spp = 4;
+    o->train_pass_spp = 8;
     o->glossy_min_alpha = 0.5f;
     o->guided_depth = 3;
     return SPK_OK;

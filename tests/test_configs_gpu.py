@@ -101,7 +101,7 @@ def _render(gs, wl, spp, guided, queue=1 << 24):
     g = None
     if guided:
         extra = dict(glossy_min_alpha=0.0, guided_depth=64) if guided == "all-glossy" else {}
-        g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=4, steps_per_pass=8, **extra)
+        g = render.Guide(batch_size=1 << 18, train_spp=64, train_pass_spp=8, steps_per_pass=8, **extra)
         gr.set_guide(g)
     gr.run(spp, sync=True)
     film = gr.film()
