@@ -3,11 +3,12 @@
 // One Adam step over a batch of B samples (x, w_i, F/p):
 //   k_guide_step_begin  batch size and record permutation for this step (1 thread)
 //   k_guide_batch       training records of the last round -> batch, F/p from their gathered radiance
-//   k_guide_fb          fused forward + backward-data per 16-sample tile: stores the one-blob code,
-//                       hidden activations and the back-propagated deltas (bf16, unit-major)
+//   k_guide_fb          fused forward + backward-data per 16-sample tile: stores the hidden
+//                       activations and the back-propagated deltas (bf16, unit-major)
 //   k_guide_dw          weight gradients dW = delta * a^T as MFMA tiles over sample ranges
-//                       (one partial per range, no atomics)
-//   k_guide_gsum        sums the partials, and per block the squared gradient norm
+//                       (one partial per range, no atomics; layer 1 recomputes the one-blob code)
+//   k_guide_psum        folds the partials into 16 fixed groups of ranges
+//   k_guide_gsum        (norm clipping) sums the partials, and per block the squared gradient norm
 //   k_guide_gnorm       the blocks' squared norms in a fixed order (norm clipping)
 //   k_guide_adam        Adam update of the fp32 master weights (the step clipped to a bound on
 //                       its gradient norm), and rewrites the bf16 forward/backward images.
@@ -199,13 +200,11 @@ __device__ __forceinline__ void store_row(uint16_t* row, int g, const u32x4_t& c
 }
 
 struct FbHook {
-    uint16_t* erow;     // one-blob code of this sample (224 bf16)
     uint16_t* arow[4];  // hidden activations of this sample (64 bf16 each)
     int g;
     u32x4_t hs[4][2];
-    __device__ __forceinline__ void enc(int c, int, const u32x4_t& b) {
-        *reinterpret_cast<u32x4_t*>(erow + 32 * c + 8 * g) = b;
-    }
+    // the one-blob code is not stored: k_guide_dw recomputes it from the sample's inputs
+    __device__ __forceinline__ void enc(int, int, const u32x4_t&) {}
     template <int NCT>
     __device__ __forceinline__ void act_impl(int l, const u32x4_t (&h)[2][NCT]) {
         hs[l - 1][0] = h[0][0];
@@ -266,7 +265,6 @@ __global__ __launch_bounds__(256) void k_guide_fb(const uint16_t* __restrict__ w
         const float4 x0 = tb.tb[0][q], x1 = tb.tb[1][q];
         const float xin[7] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z};
         FbHook hook;
-        hook.erow = tb.E + (size_t)q * kGEnc;
 #pragma unroll
         for (int l = 0; l < 4; l++) hook.arow[l] = tb.act + ((size_t)l * ld + q) * 64;
         hook.g = g;
@@ -400,7 +398,7 @@ __global__ __launch_bounds__(64) void k_guide_dw(TrainBufs tb, const GuideState*
     const int nkt = ncols / 16;
     const size_t ld = tb.ld;
     const uint16_t* A = tb.dz + (size_t)(layer - 1) * ld * 64;                       // [q][64]
-    const uint16_t* B = layer == 1 ? tb.E + kbase : tb.act + (size_t)(layer - 2) * ld * 64;  // [q][K] from kbase
+    const uint16_t* B = tb.act + (size_t)(layer == 1 ? 0 : layer - 2) * ld * 64;  // [q][64] (layers 2..5)
     const uint32_t q_lo = blockIdx.y * cols_per_split, q_hi = min(tb.ncols, q_lo + cols_per_split);
     f32x4_t acc[4][4];
     float bsum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -416,7 +414,17 @@ __global__ __launch_bounds__(64) void k_guide_dw(TrainBufs tb, const GuideState*
             const int qs = e >> 3, ch = e & 7;
             const u32x4_t va = *reinterpret_cast<const u32x4_t*>(A + (size_t)(q0 + qs) * 64 + 8 * ch);
             u32x4_t vb = u32x4_t{0u, 0u, 0u, 0u};
-            if (8 * ch < ncols) vb = *reinterpret_cast<const u32x4_t*>(B + (size_t)(q0 + qs) * K + 8 * ch);
+            if (8 * ch < ncols) {
+                if (layer == 1) {
+                    // code bins col .. col + 7 of the sample: input col / 32, bins col % 32 ..,
+                    // computed as the forward pass of k_guide_fb computed them
+                    const int col = kbase + 8 * ch, c = col >> 5;
+                    const float* xr = reinterpret_cast<const float*>(tb.tb[c >> 2] + (q0 + qs));
+                    vb = blob8(xr[c & 3], (col >> 3) & 3);
+                } else {
+                    vb = *reinterpret_cast<const u32x4_t*>(B + (size_t)(q0 + qs) * K + 8 * ch);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const uint32_t wa = va[j >> 1], wbv = vb[j >> 1];
@@ -501,6 +509,22 @@ __global__ __launch_bounds__(256) void k_guide_gsum(float* partial, int splits, 
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
     __syncthreads();
     if (threadIdx.x == 0) gnorm_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// The weight-gradient partials of a step (one row per range of 1024 samples) folded into
+// kPsumGroups rows, each the sum of a fixed run of consecutive rows, so that Adam sums 16
+// numbers per parameter instead of one per range (a 2^18-sample step has 256 ranges: summed by
+// one thread per parameter they held the Adam launch to ~65 us of dependent loads).  Fixed
+// order, no atomics: a step stays bit-reproducible.
+__global__ __launch_bounds__(256) void k_guide_psum(const float* __restrict__ partial, int splits,
+                                                    const GuideState* __restrict__ gs, float* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kGParams || !gs->active) return;
+    const int per = (splits + kPsumGroups - 1) / kPsumGroups;
+    const int k0 = (int)blockIdx.y * per, k1 = min(splits, k0 + per);
+    float s = 0.0f;
+    for (int k = k0; k < k1; k++) s += partial[(size_t)k * kGParams + p];
+    out[(size_t)blockIdx.y * kGParams + p] = s;
 }
 
 // The blocks' partials summed in a fixed order (no float atomics: training stays
@@ -629,14 +653,21 @@ int launch_guide_step(const GuideBufsDev& gb, const TrainBufs& tb, const GuideNe
     const uint32_t splits = (tb.ncols + cps - 1) / cps;
     hipLaunchKernelGGL(k_guide_dw, dim3(8, splits), dim3(64), 0, st, tb, (const GuideState*)gb.gs, gb.partial, cps);
     const bool clip = kGuideClip > 0.0f && !grad_out;
+    const float* red = gb.partial;  // the rows Adam sums
+    int nred = (int)splits;
     if (clip) {
         hipLaunchKernelGGL(k_guide_gsum, dim3(kGnormBlocks), dim3(256), 0, st, gb.partial, (int)splits,
                            (const GuideState*)gb.gs, gb.gnorm_part);
         hipLaunchKernelGGL(k_guide_gnorm, dim3(1), dim3(64), 0, st, (const float*)gb.gnorm_part, gb.gs);
+        nred = 1;
+    } else if (splits > 1) {
+        hipLaunchKernelGGL(k_guide_psum, dim3((kGParams + 255) / 256, kPsumGroups), dim3(256), 0, st,
+                           (const float*)gb.partial, (int)splits, (const GuideState*)gb.gs, gb.psum);
+        red = gb.psum;
+        nred = kPsumGroups;
     }
-    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gb.params, gb.m, gb.v,
-                       (const float*)gb.partial, clip ? 1 : (int)splits, (const GuideState*)gb.gs, cfg, grad_out, gb.wf,
-                       gb.wb, gb.bias);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGParams + 255) / 256), dim3(256), 0, st, gb.params, gb.m, gb.v, red, nred,
+                       (const GuideState*)gb.gs, cfg, grad_out, gb.wf, gb.wb, gb.bias);
     SPK_CHECK_HIP(hipGetLastError());
     return SPK_OK;
 }

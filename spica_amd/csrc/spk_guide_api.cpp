@@ -24,6 +24,7 @@ spk::GuideBufsDev spk_guide::bufs() const {
     b.wb = wb.as<uint16_t>();
     b.bias = bias.as<float>();
     b.partial = partial.as<float>();
+    b.psum = psum.as<float>();
     b.gnorm_part = gnorm_part.as<float>();
     b.gs = state.as<GuideState>();
     b.grid = cus;
@@ -33,7 +34,6 @@ spk::GuideBufsDev spk_guide::bufs() const {
 spk::TrainBufs spk_guide::train_bufs(uint32_t ncols) const {
     TrainBufs t;
     for (int k = 0; k < 3; k++) t.tb[k] = tb[k].as<float4>();
-    t.E = E.as<uint16_t>();
     t.act = act.as<uint16_t>();
     t.dz = dz.as<uint16_t>();
     t.ld = bcap;
@@ -200,12 +200,13 @@ int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out) {
     int rc;
     if ((rc = g->params.alloc(kGParams * 4)) || (rc = g->m.alloc(kGParams * 4)) || (rc = g->v.alloc(kGParams * 4)) ||
         (rc = g->wf.alloc(kGFwdSize * 2)) || (rc = g->wb.alloc(kGBwdSize * 2)) || (rc = g->bias.alloc(kGBiasSize * 4)) ||
-        (rc = g->partial.alloc(splits * kGParams * 4)) || (rc = g->state.alloc(sizeof(GuideState))) ||
+        (rc = g->partial.alloc(splits * kGParams * 4)) || (rc = g->psum.alloc((size_t)kPsumGroups * kGParams * 4)) ||
+        (rc = g->state.alloc(sizeof(GuideState))) ||
         (rc = g->grad.alloc(kGParams * 4)) || (rc = g->gnorm_part.alloc(kGnormBlocks * 4)))
         return rc;
     for (int k = 0; k < 3; k++)
         if ((rc = g->tb[k].alloc(B * 16))) return rc;
-    if ((rc = g->E.alloc(B * kGEnc * 2)) || (rc = g->act.alloc(B * 4 * 64 * 2)) || (rc = g->dz.alloc(B * 5 * 64 * 2)))
+    if ((rc = g->act.alloc(B * 4 * 64 * 2)) || (rc = g->dz.alloc(B * 5 * 64 * 2)))
         return rc;
     // padding of the bf16 images and biases stays zero
     SPK_CHECK_HIP(hipMemset(g->wf.p, 0, kGFwdSize * 2));
@@ -226,8 +227,8 @@ int spk_guide_destroy(spk_guide* g) {
         (void)hipStreamSynchronize(g->stream);
         (void)hipStreamDestroy(g->stream);
     }
-    DevBuf* b[] = {&g->params, &g->m, &g->v, &g->wf, &g->wb, &g->bias, &g->partial, &g->state, &g->grad,
-                   &g->gnorm_part, &g->tb[0], &g->tb[1], &g->tb[2], &g->E, &g->act, &g->dz};
+    DevBuf* b[] = {&g->params, &g->m, &g->v, &g->wf, &g->wb, &g->bias, &g->partial, &g->psum, &g->state, &g->grad,
+                   &g->gnorm_part, &g->tb[0], &g->tb[1], &g->tb[2], &g->act, &g->dz};
     for (DevBuf* x : b) x->release();
     delete g;
     return SPK_OK;

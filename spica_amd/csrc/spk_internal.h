@@ -204,8 +204,8 @@ int launch_relmse_pixels(const RenderBufs& b, int64_t n, const int32_t* pixels, 
 
 // ---- path guiding (spk_guide.hip) -------------------------------------------------
 struct TrainBufs {
-    float4* tb[3];      // batch: (x0..x3), (x4..x6, F/p), (direction)
-    uint16_t* E;        // one-blob code [ld][224]
+    float4* tb[3];      // batch: (x0..x3), (x4..x6, F/p), (direction); the one-blob code is
+                        // recomputed from x where it is needed, not stored
     uint16_t* act;      // hidden activations [4][ld][64], slot order
     uint16_t* dz;       // deltas: layers 1..4 [4][ld][64] slot order, layer 5 [ld][64] natural order
     uint32_t ld;        // samples per array (batch capacity)
@@ -223,11 +223,13 @@ struct GuideBufsDev {
     uint16_t* wb;
     float* bias;
     float* partial;     // [splits][kGParams]
+    float* psum;        // [kPsumGroups][kGParams]: partial folded in fixed groups of ranges
     float* gnorm_part;  // [kGnormBlocks] squared-norm partials of one step's gradient
     GuideState* gs;
     int grid;           // persistent blocks for the training kernel (compute units)
 };
 constexpr uint32_t kGuideColsPerSplit = 1024;
+constexpr int kPsumGroups = 16;
 constexpr int kGnormBlocks = (kGParams + 255) / 256;
 
 int launch_guide_images(const float* params, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t st);
@@ -250,8 +252,8 @@ struct spk_guide {
     int device = 0;
     hipStream_t stream = nullptr;       // for the stand-alone entry points
     spk_guide_opts opts;
-    spk::DevBuf params, m, v, wf, wb, bias, partial, state, grad, gnorm_part;
-    spk::DevBuf tb[3], E, act, dz;
+    spk::DevBuf params, m, v, wf, wb, bias, partial, psum, state, grad, gnorm_part;
+    spk::DevBuf tb[3], act, dz;
     uint32_t bcap = 0;                  // batch capacity (samples)
     int cus = 256;                      // compute units of the device
     int64_t steps = 0;                  // Adam steps launched
