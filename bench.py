@@ -1,6 +1,6 @@
 """Benchmark: plain wavefront path tracing of the 512-sphere field on one MI355X.
 
-Workload: seed-1 sphere field, 512 spheres x 640 triangles
+Workload (the plain tracer at scale): seed-1 sphere field, 512 spheres x 640 triangles
 on a ground quad (327,686 triangles), two quad lights, 1920x1080, maxDepth 16,
 Russian roulette after bounce 3, Sobol sampler.  One step = one sample per pixel
 over the whole frame (2,073,600 paths), i.e. one pass of the hot path; the full

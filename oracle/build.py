@@ -8,7 +8,7 @@
   stb / tinyobjloader / tinyxml2, which are not vendored, and are left out.  The
   Image symbols that Film and MipMap reference stay unresolved and are never
   called, so the library is loaded with RTLD_LAZY.  Built only where
-  /root/reference exists; the GPU box never needs it.
+  /root/reference exists; the product, its GPU tests and the benchmark never load it.
 """
 import os
 import subprocess
