@@ -248,7 +248,7 @@ def guide_roofline(gs3, w3, queue, block, guide_cfg):
                 unit="TFLOP/s", frac=round(achieved / BF16_PEAK_TFLOPS, 4),
                 training=dict(flop_per_sample=GUIDE_TRAIN_FLOP, batch=b, step_us=round(step_s * 1e6, 1),
                               achieved=round(train_tflops, 2), frac=round(train_tflops / BF16_PEAK_TFLOPS, 4),
-                              kernels="k_guide_fb + k_guide_dw + k_guide_adam"))
+                              kernels="k_guide_fb + k_guide_dw + k_guide_psum + k_guide_adam"))
 
 
 def c3_leg(args, c3_ref):
