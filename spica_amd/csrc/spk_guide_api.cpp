@@ -169,12 +169,7 @@ int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out) {
     clear_error();
     if (!out) return SPK_ERR_INVALID;
     *out = nullptr;
-    int ndev = 0;
-    SPK_CHECK_HIP(hipGetDeviceCount(&ndev));
-    if (device < 0 || device >= ndev) {
-        set_error("spk_guide_create: device %d not present", device);
-        return SPK_ERR_INVALID;
-    }
+    // the options first (no device needed to reject them)
     spk_guide_opts o;
     spk_guide_opts_default(&o);
     if (opts) o = *opts;
@@ -186,6 +181,12 @@ int spk_guide_create(int device, const spk_guide_opts* opts, spk_guide** out) {
         !(o.guide_fraction < 1.0f) || !(o.beta1 >= 0.0f && o.beta1 < 1.0f) || !(o.beta2 >= 0.0f && o.beta2 < 1.0f) ||
         !(o.glossy_min_alpha >= 0.0f && o.glossy_min_alpha < 1.0e30f) || o.guided_depth < 1) {
         set_error("spk_guide_create: bad options");
+        return SPK_ERR_INVALID;
+    }
+    int ndev = 0;
+    SPK_CHECK_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        set_error("spk_guide_create: device %d not present", device);
         return SPK_ERR_INVALID;
     }
     std::unique_ptr<spk_guide> g(new spk_guide());

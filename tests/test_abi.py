@@ -129,3 +129,20 @@ def test_guide_option_defaults():
         assert "glossy_min_alpa" in str(e)
     else:
         raise AssertionError("a misspelt guide option was accepted")
+
+
+def test_guide_options_are_checked():
+    """Malformed guide options are rejected before any device is touched."""
+    from spica_amd import render
+
+    lib = _abi.load()
+    h = ctypes.c_void_p()
+    bad = [dict(batch_size=100), dict(batch_size=0), dict(train_pass_spp=0), dict(learning_rate=0.0),
+           dict(guide_fraction=1.0), dict(beta2=1.0), dict(beta1=-0.1), dict(glossy_min_alpha=-0.5),
+           dict(glossy_min_alpha=float("nan")), dict(guided_depth=0), dict(steps_per_pass=-1)]
+    for kw in bad:
+        o = render.guide_opts(**kw)
+        rc = lib.spk_guide_create(0, ctypes.byref(o), ctypes.byref(h))
+        msg = lib.spk_last_error()
+        assert rc == _abi.SPK_ERR_INVALID and (b"bad options" in msg or b"batch_size" in msg), (kw, rc, msg)
+        assert not h.value

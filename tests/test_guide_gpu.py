@@ -275,3 +275,27 @@ def test_guided_render_is_deterministic(gpu):
     print(f"guided film sums {films[0].sum():.9g} {films[1].sum():.9g}")
     assert np.array_equal(films[0], films[1])
     gs.close()
+
+
+def test_guided_vertex_policy(gpu):
+    """guided_depth caps the guided bounces (at most that many guided vertices per path) and
+    glossy_min_alpha decides whether C3's GGX spheres (alpha 0.1-0.3) are guided."""
+    from spica_amd import configs, render
+    wl = configs.c3_occluded_glossy(width=160, height=90)
+    gs = render.GpuScene(wl.scene)
+    spp, paths = 32, 32 * 160 * 90
+    counts = {}
+    for name, kw in (("depth1", dict(guided_depth=1)), ("default", {}),
+                     ("all", dict(guided_depth=64, glossy_min_alpha=0.0))):
+        gr = render.GpuRender(gs, wl.camera, wl.render_desc(spp=spp), queue_size=1 << 22, block_size=64)
+        g = render.Guide(batch_size=1 << 14, train_spp=16, train_pass_spp=8, steps_per_pass=2, **kw)
+        gr.set_guide(g)
+        gr.run(spp)
+        counts[name] = gr.stats()["guided_vertices"]
+        gr.close()
+        g.close()
+    print(f"guided vertices per path: {', '.join(f'{k} {v / paths:.3f}' for k, v in counts.items())}")
+    assert 0 < counts["depth1"] <= paths
+    assert counts["depth1"] < counts["default"] <= 3 * paths
+    assert counts["default"] < counts["all"]
+    gs.close()
