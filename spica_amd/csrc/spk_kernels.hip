@@ -1671,7 +1671,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(shade_wav
                         }
                     }
                     // Continuation (path.cc:83-94) and Russian roulette (path.cc:116-121); with a
-                    // guide the non-specular vertices go to the guide queue instead.
+                    // guide the vertices it may draw for (guide_eligible, within guide_depth
+                    // bounces) go to the guide queue instead.
                     if (direct) {
                         // specularReflect (integrator.cc:112-130) while depth + 1 < maxDepth, where
                         // spica's depth grows by 2 per specular bounce (Li -> specularReflect(depth + 1)
