@@ -329,7 +329,9 @@ def main():
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy-scene (C4) leg")
-    ap.add_argument("--heavy-spp", type=int, default=256, help="samples per pixel of the C4 relMSE renders")
+    ap.add_argument("--heavy-spp", type=int, default=2048,
+                    help="samples per pixel of the C4 relMSE renders (the configured 2048; at <= 512 the guided "
+                         "batch also runs with guide seeds 2 and 3)")
     ap.add_argument("--c3-seeds", type=int, nargs="+", default=[7, 8, 9, 10],
                     help="render seeds of the C3 time-to-relMSE leg")
     ap.add_argument("--copy-probe", type=int, default=0,
