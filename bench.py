@@ -329,6 +329,9 @@ def main():
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batch leg")
     ap.add_argument("--no-heavy", action="store_true", help="skip the heavy-scene (C4) leg")
+    ap.add_argument("--no-batch-graph", action="store_true",
+                    help="skip the C2 batch as one whole-batch graph launch (a graph of ~50k kernels that "
+                         "rocprofv3's kernel trace does not accept)")
     ap.add_argument("--heavy-spp", type=int, default=2048,
                     help="samples per pixel of the C4 relMSE renders (the configured 2048; at <= 512 the guided "
                          "batch also runs with guide seeds 2 and 3)")
@@ -505,28 +508,31 @@ def main():
                                    relmse=float(f"{float(((m - refm) ** 2 / (refm ** 2 + 1e-2)).mean()):.4g}"))
             batch["relmse"] = dict(images=rel, eps=1e-2,
                                    reference=f"{c2_ref['spp']}-spp float64 CPU render, stratified pixels per image")
-        # the same batch as one whole-batch hipGraph launch: classic passes (a fixed launch sequence
-        # per pass, so the graph holds every pass), films checked against the regenerating run's
-        import numpy as np
+        if args.no_batch_graph:
+            b.close()
+        else:
+            # the same batch as one whole-batch hipGraph launch: classic passes (a fixed launch sequence
+            # per pass, so the graph holds every pass), films checked against the regenerating run's
+            import numpy as np
 
-        films = [b.film(i) for i in (0, 1)]
-        b.close()
-        bg = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=bq, block_size=args.block,
-                          graph=2, regen=False)
-        bg.run(sync=True)  # capture and first launch
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        bg.run(sync=True)
-        torch.cuda.synchronize()
-        gdt = time.perf_counter() - t0
-        same = all(np.array_equal(x, y) for i, f in zip((0, 1), films) for x, y in zip(bg.film(i), f))
-        ginfo = bg.info()
-        bg.close()
-        batch["whole_batch_graph"] = dict(seconds=round(gdt, 3), mrays_per_s=round(brays / gdt / 1e6, 3),
-                                          passes=int(ginfo["passes"]), queue=int(ginfo["queue"]),
-                                          films_equal_images_0_1=bool(same),
-                                          note="classic passes, the whole batch captured as one graph and launched once")
-        log(f"bench: C2 whole-batch graph {batch['whole_batch_graph']}")
+            films = [b.film(i) for i in (0, 1)]
+            b.close()
+            bg = render.Batch(items, max_depth=wb.max_depth, sampler=wb.sampler, queue_size=bq, block_size=args.block,
+                              graph=2, regen=False)
+            bg.run(sync=True)  # capture and first launch
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bg.run(sync=True)
+            torch.cuda.synchronize()
+            gdt = time.perf_counter() - t0
+            same = all(np.array_equal(x, y) for i, f in zip((0, 1), films) for x, y in zip(bg.film(i), f))
+            ginfo = bg.info()
+            bg.close()
+            batch["whole_batch_graph"] = dict(seconds=round(gdt, 3), mrays_per_s=round(brays / gdt / 1e6, 3),
+                                              passes=int(ginfo["passes"]), queue=int(ginfo["queue"]),
+                                              films_equal_images_0_1=bool(same),
+                                              note="classic passes, the whole batch captured as one graph and launched once")
+            log(f"bench: C2 whole-batch graph {batch['whole_batch_graph']}")
         for g in scenes:
             g.close()
         # the same batch over trees built on the device: the time to the first pixel
