@@ -191,7 +191,7 @@ typedef struct spk_guide_opts {
     int32_t steps_per_pass;  /* Adam steps after each training round (default 8) */
     int32_t train_spp;       /* samples per pixel of online training (default 64) */
     float learning_rate;     /* default 1e-2 */
-    float beta1, beta2, epsilon;  /* default 0.9, 0.95, 1e-8 */
+    float beta1, beta2, epsilon;  /* default 0.95, 0.95, 1e-8 */
     float guide_fraction;    /* probability of sampling the proposal at a vertex (default 0.5) */
     int32_t train_pass_spp;  /* samples per pixel per training round (default 8: with 8 steps,
                               * 64 Adam steps over the 64 training spp); each round keeps

@@ -122,7 +122,7 @@ def test_guide_option_defaults():
     o = render.guide_opts()
     assert (o.batch_size, o.train_spp, o.train_pass_spp, o.steps_per_pass) == (1 << 18, 64, 8, 8)
     assert abs(o.learning_rate - 1e-2) < 1e-9 and o.guide_fraction == 0.5 and o.glossy_min_alpha == 0.5
-    assert abs(o.beta2 - 0.95) < 1e-7 and o.guided_depth == 3
+    assert abs(o.beta1 - 0.95) < 1e-7 and abs(o.beta2 - 0.95) < 1e-7 and o.guided_depth == 3
     try:
         render.guide_opts(glossy_min_alpa=0.0)
     except TypeError as e:
